@@ -1,0 +1,101 @@
+/* ptl.h -- C ABI of the MI355X code generator (libptl_hip.so).
+ *
+ * Drop-in boundary for the reference's rv_generator
+ * (include/codegen/rv_generator.hpp:19-74): a pareas-ast DepthTree goes in,
+ * RISC-V instruction words and the function table come out, bit-exact to
+ * rv_generator_avx run with one thread.  Plain pointers and sizes only.
+ *
+ * Status codes: 0 = PTL_OK; negative values are errors, ptl_strerror() names them.
+ * A context owns one HIP stream and a pooled device arena; it is not
+ * thread-safe, but separate contexts may be used from separate threads.
+ */
+#ifndef PTL_H
+#define PTL_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PTL_OK = 0,
+    PTL_ERR_ARG = -1,          /* null pointer or inconsistent sizes */
+    PTL_ERR_INVALID_AST = -2,  /* arrays violate the DepthTree contract */
+    PTL_ERR_EMPTY = -3,        /* no functions: the reference takes max() of an empty range */
+    PTL_ERR_NOMEM = -4,        /* arena or host allocation failed */
+    PTL_ERR_HIP = -5,          /* a HIP runtime call failed */
+    PTL_ERR_UNSUPPORTED = -6,  /* input reaches a path the reference leaves undefined */
+    PTL_ERR_ADMISSION = -7,    /* serving: request larger than the arena can ever hold */
+};
+
+/* Input: exactly the six DepthTree arrays (pareas-ast include/codegen/depthtree.hpp:12-58,
+ * built by depthtree.cpp:11-100), host memory. Root: parent -1, child_idx -1. */
+typedef struct {
+    const uint8_t *node_types;    /* pareas NodeType (astnode.hpp:11-56) */
+    const uint8_t *result_types;  /* pareas DataType (datatype.hpp:7-14) */
+    const int32_t *parents;
+    const int32_t *depth;
+    const int32_t *child_idx;
+    const uint32_t *node_data;
+    uint64_t n_nodes;
+    uint32_t max_depth;           /* DepthTree::maxDepth() */
+} ptl_ast;
+
+/* Output, owned by the context and valid until the next call on it or
+ * ptl_destroy.  words == rv_generator::get_instructions()
+ * (rv_generator_avx.cpp:82-84); the function table is the protected
+ * function_ids / func_starts / func_ends / function_sizes members
+ * (rv_generator.hpp:46-51) in post-order function order. */
+typedef struct {
+    const uint32_t *words;
+    uint64_t n_words;
+    const uint32_t *func_ids, *func_starts, *func_ends, *func_sizes;
+    uint32_t n_funcs;
+    uint64_t stage_ns[7];         /* preprocess, isn_cnt, isn_gen, optimize, regalloc, fix_jumps, postprocess */
+    uint64_t n_words_gen;         /* W0: words after isn_gen */
+    uint64_t n_words_alloc;       /* W1: words after regalloc */
+    uint32_t opt_iterations;      /* optimize fixpoint launches */
+    uint32_t tie_slots;           /* word slots written by two nodes of equal depth */
+} ptl_result;
+
+typedef struct ptl_ctx ptl_ctx;
+
+/* Creates a context on HIP device `hip_device` with a device arena of
+ * `arena_bytes` (0: grow on demand). */
+int ptl_create(ptl_ctx **out, int hip_device, uint64_t arena_bytes);
+void ptl_destroy(ptl_ctx *ctx);
+const char *ptl_strerror(int status);
+
+/* Replaces rv_generator_avx(tree, 1, 0, false, 0) + process() + the getters
+ * (src/main.cpp:121-157): host arrays in, host result out, synchronous. */
+int ptl_compile(ptl_ctx *ctx, const ptl_ast *in, ptl_result *out);
+
+/* Device-resident variants for callers that keep trees in HBM.
+ * ptl_upload copies a tree into the context's resident slot `slot`
+ * (0 <= slot < 64); ptl_compile_resident runs the pipeline on it.  With
+ * copy_out == 0 the result's host pointers are NULL and only the sizes are
+ * filled; ptl_result_device then yields the device pointer of the words. */
+int ptl_upload(ptl_ctx *ctx, int slot, const ptl_ast *in);
+int ptl_compile_resident(ptl_ctx *ctx, int slot, int copy_out, ptl_result *out);
+const uint32_t *ptl_result_device(ptl_ctx *ctx);
+
+/* Batched compile: n_programs trees concatenated node-wise; program p covers
+ * nodes [prog_node_offsets[p], prog_node_offsets[p+1]) and its parents are
+ * local to that range.  out[p] points into the context's result storage. */
+int ptl_compile_batch(ptl_ctx *ctx, const ptl_ast *concat, const int64_t *prog_node_offsets,
+                      const uint32_t *max_depths, uint32_t n_programs, ptl_result *out);
+
+/* Host-side stage dumps for differential testing (stage 1..7 boundary arrays
+ * of the last compile): name is one of "types", "rtypes", "node_data", "locs",
+ * "func_ids", "func_starts", "func_ends", "func_sizes", "instr", "rd", "rs1",
+ * "rs2", "jt", "used", "sym_reg", "sym_sw", "sorted".  Copies up to cap
+ * elements (u32) into dst and returns the element count, or a negative status. */
+int64_t ptl_debug_dump(ptl_ctx *ctx, const char *name, uint32_t *dst, uint64_t cap);
+
+/* Runs the pipeline only up to and including `stage` (1..7) on the next
+ * ptl_compile (0 = all stages); for stage-boundary parity tests. */
+int ptl_debug_stop_after(ptl_ctx *ctx, int stage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,60 @@
+"""ctypes declarations of the C ABI in include/ptl.h."""
+from __future__ import annotations
+
+import ctypes as C
+
+STAGES = ("preprocess", "isn_cnt", "isn_gen", "optimize", "regalloc", "fix_jumps", "postprocess")
+
+PTL_OK = 0
+ERRORS = {
+    -1: "PTL_ERR_ARG", -2: "PTL_ERR_INVALID_AST", -3: "PTL_ERR_EMPTY", -4: "PTL_ERR_NOMEM",
+    -5: "PTL_ERR_HIP", -6: "PTL_ERR_UNSUPPORTED", -7: "PTL_ERR_ADMISSION",
+}
+
+# every symbol include/ptl.h declares
+EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_upload", "ptl_compile_resident",
+           "ptl_result_device", "ptl_compile_batch", "ptl_debug_dump", "ptl_debug_stop_after")
+
+
+class PtlAst(C.Structure):
+    _fields_ = [
+        ("node_types", C.c_void_p), ("result_types", C.c_void_p), ("parents", C.c_void_p),
+        ("depth", C.c_void_p), ("child_idx", C.c_void_p), ("node_data", C.c_void_p),
+        ("n_nodes", C.c_uint64), ("max_depth", C.c_uint32),
+    ]
+
+
+class PtlResult(C.Structure):
+    _fields_ = [
+        ("words", C.POINTER(C.c_uint32)), ("n_words", C.c_uint64),
+        ("func_ids", C.POINTER(C.c_uint32)), ("func_starts", C.POINTER(C.c_uint32)),
+        ("func_ends", C.POINTER(C.c_uint32)), ("func_sizes", C.POINTER(C.c_uint32)),
+        ("n_funcs", C.c_uint32),
+        ("stage_ns", C.c_uint64 * 7),
+        ("n_words_gen", C.c_uint64), ("n_words_alloc", C.c_uint64),
+        ("opt_iterations", C.c_uint32), ("tie_slots", C.c_uint32),
+    ]
+
+
+def bind(lib):
+    lib.ptl_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_uint64]
+    lib.ptl_create.restype = C.c_int
+    lib.ptl_destroy.argtypes = [C.c_void_p]
+    lib.ptl_strerror.argtypes = [C.c_int]
+    lib.ptl_strerror.restype = C.c_char_p
+    lib.ptl_compile.argtypes = [C.c_void_p, C.POINTER(PtlAst), C.POINTER(PtlResult)]
+    lib.ptl_compile.restype = C.c_int
+    lib.ptl_upload.argtypes = [C.c_void_p, C.c_int, C.POINTER(PtlAst)]
+    lib.ptl_upload.restype = C.c_int
+    lib.ptl_compile_resident.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(PtlResult)]
+    lib.ptl_compile_resident.restype = C.c_int
+    lib.ptl_result_device.argtypes = [C.c_void_p]
+    lib.ptl_result_device.restype = C.c_void_p
+    lib.ptl_compile_batch.argtypes = [C.c_void_p, C.POINTER(PtlAst), C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.POINTER(PtlResult)]
+    lib.ptl_compile_batch.restype = C.c_int
+    lib.ptl_debug_dump.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_uint64]
+    lib.ptl_debug_dump.restype = C.c_int64
+    lib.ptl_debug_stop_after.argtypes = [C.c_void_p, C.c_int]
+    lib.ptl_debug_stop_after.restype = C.c_int
+    return lib

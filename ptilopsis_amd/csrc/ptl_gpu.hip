@@ -1,0 +1,1028 @@
+// MI355X (gfx950) implementation of the reference's data-parallel RISC-V code
+// generator: rv_generator_avx with one thread (src/codegen/rv_generator_avx.cpp),
+// stage for stage, bit-exact.
+//
+// Every stage is integer gather/scatter/scan work bound by HBM bandwidth; no
+// stage is a dense contraction.  Arrays are structure-of-arrays in HBM; u8 for
+// node and data types, u32 elsewhere.  Stages and where their semantics come from:
+//
+//   preprocess   k_pre12 (passes 1+2, :126-266), k_pre3 (pass 3, :270-305)
+//   isn_cnt      k_scan<CountOp> (count :315-369 + exclusive scan :374-392),
+//                k_cnt_fix (:398-447), function table (:451-515)
+//   isn_gen      k_gen_regs (parent-slot propagation) then k_gen_owner /
+//                k_gen_write (:584-886).  The reference walks depth levels
+//                bottom-up, but every parent slot has one writer and is read
+//                only by its parent, so both passes run over all nodes at once;
+//                word slots written by more than one node go to the writer the
+//                reference writes last: the shallowest node, then the later one
+//                in the reference's depth sort (its libstdc++ std::sort order,
+//                reproduced exactly by the introsort kernels below).
+//   optimize     k_opt_init + k_opt_iter (:891-1016), a chaotic in-place
+//                iteration of the same monotone operator; it reaches the same
+//                greatest fixpoint as the reference's two-copy iteration.
+//   regalloc     k_ra_walk (functions not in lane 0 of an 8-function group:
+//                independent), k_ra_lane0 (lane-0 functions in step lockstep:
+//                they share the first 64-entry region of the reference's
+//                register_state snapshot, :1290-1354), then counts, scan, emit
+//                (:1422-1697), function table fix and frame words (:1714-1811).
+//   fix_jumps    size + scan + fused move/encode/postprocess (:1853-1993, 1995-2011).
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <chrono>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "rv_spec.h"
+#include "../../include/ptl.h"
+
+#define DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------- constants
+__constant__ uint32_t c_word[RV_NTYPES * 4 * 6];
+__constant__ uint32_t c_bits[RV_NTYPES * 4 * 6];
+__constant__ uint8_t c_dense[256];
+__constant__ uint8_t c_pareas[44];
+
+DEV uint32_t spec_i(uint32_t ty, uint32_t o, uint32_t rt) { return ((uint32_t)c_dense[ty] * 4u + o) * 6u + rt; }
+DEV uint32_t size_of(uint32_t ty, uint32_t rt) { return (c_bits[spec_i(ty, 0, rt)] >> 19) & 7u; }
+
+// error flags raised by kernels (bit set in ctx->d_flags)
+enum : uint32_t { EF_OOB = 1, EF_XFUNC = 2, EF_CALLPATH = 4 };
+
+// ------------------------------------------------------------------ helpers
+DEV uint32_t ffz_gcc(uint64_t v) {
+    // include/utils.hpp:25-45 as compiled by GCC: __builtin_ffs(int) of ~val sees 32 bits.
+    uint32_t low = (uint32_t)(~v);
+    return low == 0 ? 64u : (uint32_t)__builtin_ctz(low);
+}
+
+template <typename T>
+__global__ void k_fill(T *p, uint64_t n, T v) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_set_u8(uint8_t *p, uint8_t v) { *p = v; }
+
+static inline uint32_t grid_for(uint64_t n, uint32_t block) {
+    uint64_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    return (uint32_t)std::min<uint64_t>(g, 0x7FFFFFFFull);
+}
+
+// --------------------------------------------------- decoupled look-back scan
+// Exclusive u32 scan over n elements.  Each 256-thread workgroup takes a 2048
+// element tile (dynamic tile ids give forward progress), scans it with wave64
+// shuffles and LDS, and chains tiles through a status word per tile
+// (bit 32: aggregate, bit 33: inclusive prefix).  Op::value(i) produces the
+// element, Op::emit(i, exclusive, value) consumes the result.
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr uint64_t SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+template <class Op>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(Op op, uint64_t n, uint64_t *status, uint32_t *tile_ctr,
+                                                      uint32_t *total) {
+    __shared__ uint32_t s_tile, s_prefix;
+    __shared__ uint32_t s_wsum[SCAN_BLOCK / 64];
+    if (threadIdx.x == 0) s_tile = atomicAdd(tile_ctr, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        uint64_t i = base + k;
+        v[k] = i < n ? op.value(i) : 0u;
+        sum += v[k];
+    }
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t x = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) s_wsum[wid] = x;
+    __syncthreads();
+    uint32_t wpre = 0, tsum = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+        if ((uint32_t)w < wid) wpre += s_wsum[w];
+        tsum += s_wsum[w];
+    }
+    if (threadIdx.x == 0) {
+        uint32_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&status[0], (2ull << 32) | tsum, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&status[tile], (1ull << 32) | tsum, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t p = (int64_t)tile - 1;
+            for (;;) {
+                uint64_t s = __hip_atomic_load(&status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t flag = (uint32_t)(s >> 32);
+                if (flag == 0) { __builtin_amdgcn_s_sleep(1); continue; }
+                excl += (uint32_t)s;
+                if (flag == 2) break;
+                --p;
+            }
+            __hip_atomic_store(&status[tile], (2ull << 32) | (excl + tsum), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = excl;
+        if ((uint64_t)(tile + 1) * SCAN_TILE >= n && total) *total = excl + tsum;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + wpre + (x - sum);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        uint64_t i = base + k;
+        if (i < n) op.emit(i, run, v[k]);
+        run += v[k];
+    }
+}
+
+// ----------------------------------------------------------- device state
+struct Dev {
+    // inputs
+    const uint8_t *pt, *prt;
+    const int32_t *par, *depth, *cidx;
+    const uint32_t *ndin;
+    uint64_t n;
+    uint32_t maxd;
+    // preprocess / isn_cnt
+    uint8_t *ty8, *rt8;
+    uint32_t *nd;
+    uint32_t *loc;
+    uint32_t *decl;          // func_decl node indices, F
+    uint32_t *fids, *fstarts, *fsizes, *fends;
+    uint32_t F;
+    // isn_gen
+    uint32_t *regs;          // 3n
+    uint32_t *rank;          // n (sorted position of each node)
+    uint64_t *owner;         // W0
+    uint32_t W0;
+    uint32_t *w0_instr, *w0_rd, *w0_rs1, *w0_rs2, *w0_jt;
+    // optimize
+    uint8_t *used, *sstore, *usedi;
+    // regalloc
+    uint8_t *sym_reg, *sym_sw;
+    uint32_t *ra_off;
+    uint32_t W1;
+    uint32_t *w1_instr, *w1_rd, *w1_rs1, *w1_rs2, *w1_jt;
+    // fix_jumps
+    uint32_t *fj_off;
+    uint32_t W2;
+    uint32_t *words;
+    uint32_t *flags;
+    uint32_t *counter;
+};
+
+// ================================================================ preprocess
+DEV void classify(uint32_t &ty, uint32_t rt, uint32_t ci, uint32_t &nd) {
+    if ((ty & 0xF0u) != 0xF0u) return;
+    if (rt == DT_FLOAT || rt == DT_FLOAT_REF) {
+        bool comp = (int32_t)nd > 7;
+        uint32_t reg = ci - 8u;
+        bool stk = (int32_t)reg > 7 && comp;
+        bool asint = !stk && comp;
+        if (stk) ty |= 2u;
+        if (asint) ty |= 1u;
+        if (stk) reg -= 8u;
+        if (comp) nd = reg;
+    } else {
+        uint32_t id = nd, fa = ci - id;
+        uint32_t reg = id + (fa - ((int32_t)fa > 7 ? 8u : 0u));
+        uint32_t osm = ((int32_t)reg > 7 ? 0xFFFFFFFFu : 0u) & id;  // ANDed with the value (D9)
+        ty |= osm & 2u;
+        reg -= 8u & osm;
+        nd = reg;
+    }
+}
+
+// Passes 1 and 2.  Pass 2's stores are gated by "the 8-node vector holds a
+// call-argument list" but not masked by it (:236-262): 8-lane groups of the
+// wave64 ballot are exactly the reference's 8-node vectors.
+__global__ __launch_bounds__(256) void k_pre12(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    uint32_t ty = 0, rt = 0, ci = 0, nd = 0;
+    bool in = i < d.n;
+    if (in) {
+        ty = c_pareas[d.pt[i]]; rt = d.prt[i]; ci = (uint32_t)d.cidx[i]; nd = d.ndin[i];
+        classify(ty, rt, ci, nd);
+    }
+    uint64_t m = __ballot(in && ty == RV_CALL_ARGS);
+    uint32_t lane = threadIdx.x & 63u;
+    bool block_has_list = ((m >> (lane & ~7u)) & 0xFFull) != 0;
+    if (!in) return;
+    if (block_has_list && i > 0) {
+        uint64_t j = i - 1;
+        uint32_t pty = c_pareas[d.pt[j]], prt = d.prt[j], pci = (uint32_t)d.cidx[j], pnd = d.ndin[j];
+        classify(pty, prt, pci, pnd);
+        if (pty == RV_CARG_STK) nd = pnd + 1u;
+        else if (pty == RV_CARG && prt == DT_FLOAT) {
+            int32_t ia = (int32_t)(pci - pnd) - 8;
+            nd = (uint32_t)(ia > 0 ? ia : 0);
+        }
+    }
+    d.ty8[i] = (uint8_t)ty;
+    d.rt8[i] = (uint8_t)rt;
+    d.nd[i] = nd;
+}
+
+// Pass 3 (:270-305): a FLOAT node marks its comparison-like parent FLOAT; the
+// parent passes it on only if it sits in a later 8-node vector than the child
+// (within one vector the scatter lands after the vector's load).
+__global__ __launch_bounds__(256) void k_pre3(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.n || d.prt[i] != DT_FLOAT) return;
+    uint64_t cur = i;
+    for (;;) {
+        int32_t p = d.par[cur];
+        if (p < 0) break;
+        if ((d.ty8[p] & 0xE0u) != 0xE0u) break;
+        d.rt8[p] = DT_FLOAT;
+        if ((cur >> 3) < ((uint64_t)p >> 3)) cur = (uint64_t)p;
+        else break;
+    }
+}
+
+// ================================================================== isn_cnt
+struct CountOp {
+    Dev d;
+    DEV uint32_t value(uint64_t i) const {
+        uint32_t ty = d.ty8[i], rt = d.rt8[i];
+        uint32_t s = size_of(ty, rt);
+        if (ty == RV_CALL_ARGS) {
+            s += 1;
+            if (i > 0 && (d.ty8[i - 1] & 0xF4u) == 0xF4u) s += (uint32_t)d.cidx[i - 1] + 1u;
+        }
+        int32_t p = d.par[i];
+        uint32_t pty = d.ty8[p != -1 ? p : 0];
+        int32_t ci = d.cidx[i];
+        bool ife = pty == RV_IF_ELSE;
+        if (ci == 0 && (ife || pty == RV_IF)) s += 1;
+        if (ci == 1 && (ife || pty == RV_WHILE)) s += 1;
+        return s;
+    }
+    DEV void emit(uint64_t i, uint32_t excl, uint32_t) const { d.loc[i] = excl; }
+};
+
+// Location fix (:398-447).  A condition child moves its control-flow parent
+// right after its own base words; call arguments then move to their slot in the
+// argument list, which always wins over the first rule at one thread.
+__global__ __launch_bounds__(256) void k_cnt_fix(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    int32_t p = d.par[i];
+    uint32_t ty = d.ty8[i];
+    uint32_t ci = (uint32_t)d.cidx[i];
+    if (p > -1) {
+        uint32_t pty = d.ty8[p];
+        bool cond = ((pty & 0xD0u) == 0xD0u) && ci == (pty & 1u);
+        if (cond && (pty & 0xF4u) != 0xF4u) d.loc[p] = d.loc[i] + size_of(ty, d.rt8[i]);
+    }
+    if ((ty & 0xF4u) == 0xF4u && p > -1) d.loc[i] = d.loc[p] + ci + 1u;
+}
+
+struct DeclOp {
+    Dev d;
+    DEV uint32_t value(uint64_t i) const { return d.ty8[i] == RV_FDECL ? 1u : 0u; }
+    DEV void emit(uint64_t i, uint32_t excl, uint32_t v) const { if (v) d.decl[excl] = (uint32_t)i; }
+};
+
+// Function table (:451-515): ids = node data, starts = previous decl location + 6.
+__global__ void k_functab(Dev d) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.F) return;
+    uint32_t dn = d.decl[k];
+    uint32_t off = 6u + d.loc[dn];
+    uint32_t prev = k ? 6u + d.loc[d.decl[k - 1]] : 0u;
+    d.fids[k] = d.nd[dn];
+    d.fstarts[k] = prev;
+    d.fsizes[k] = off - prev;
+    d.fends[k] = off;
+}
+
+// =================================================================== isn_gen
+struct Lane {
+    bool has;
+    int32_t pai;
+    uint32_t dpv, loc, word, rd, rs1, rs2, jt;
+};
+
+// One (node, offset) lane of the level loop body (:585-884).
+DEV void gen_lane(const Dev &d, uint32_t nn, uint32_t o, bool full, Lane &L) {
+    const uint32_t ty = d.ty8[nn], rt = d.rt8[nn];
+    const uint32_t si = spec_i(ty, o, rt);
+    const uint32_t b = c_bits[si];
+    const bool has = b & 1u;
+    const bool prop = has || o == 0;
+    const int32_t par = prop ? d.par[nn] : 0;
+    const bool nopar = par == -1;
+    const bool lfp = !nopar && prop;
+    const uint32_t pty = lfp ? d.ty8[par] : 0u;
+    const uint32_t ci = lfp ? (uint32_t)d.cidx[nn] : 0u;
+    const bool noncond = ((pty & 0xD0u) == 0xD0u) && !(ci == (pty & 1u));
+    bool call = noncond;
+    if ((nopar || !noncond) && prop) {
+        uint32_t ct = (b >> 2) & 3u;
+        if (ct == 1) call = true;
+        if (ct == 2 && (int32_t)rt > 1) call = true;
+    }
+    L.has = has;
+    L.pai = call ? (int32_t)((uint32_t)par * 3u + ci) : -1;
+    const uint32_t iib = o + (prop ? d.loc[nn] : 0u);
+    uint32_t rd = 0;
+    const uint32_t ndv = d.nd[nn];
+    if (has) {
+        uint32_t ct = (b >> 4) & 7u;
+        if (ct == 3) rd |= 32u;
+        if (ct == 4) rd |= 10u;
+        if (ct == 1) rd |= ndv + 10u;
+        if (ct == 2) rd |= ndv + 42u;
+        bool hov = call || (ct == 0 && ((b >> 1) & 1u));
+        if (hov) rd |= iib + 64u;
+    }
+    const bool use_rd = nopar || (!noncond && lfp);
+    const uint32_t instr_no = iib + (noncond ? ((b >> 19) & 7u) : 0u);
+    L.dpv = (use_rd ? rd : 0u) | (noncond ? instr_no : 0u);
+    if (!full || !has) { L.rd = rd; return; }
+    uint32_t iloc = iib;
+    if (o == 1 && ty == RV_IF_ELSE) iloc = d.regs[(uint64_t)nn * 3 + 1];
+    if (o == 1 && ty == RV_WHILE) iloc = d.regs[(uint64_t)nn * 3 + 2];
+    const bool add2 = ty == RV_FDECL || (o > 1 && ty == RV_FDECL_DUMMY);
+    const bool cal1 = o == 1 && ty == RV_CALL_ARGS;
+    if (add2 || cal1) iloc += 2u;
+    if (cal1 && nn > 0) {
+        uint32_t ppt = d.ty8[nn - 1];
+        if ((ppt & 0xF4u) == 0xF4u) iloc += (uint32_t)d.cidx[nn - 1];
+    }
+    uint32_t word = c_word[si];
+    const uint32_t cc = (b >> 13) & 7u;
+    const uint32_t nd4 = ndv * 4u;
+    if (cc == 1) word |= ndv & 0xFFFFF000u;
+    if (cc == 2) word |= (ndv & 0xFFFu) << 20;
+    if (cc == 3) word |= (0u - (nd4 + 8u)) << 20;
+    if (cc == 4) word |= nd4 << 20;
+    if (cc == 5) word |= (0u - nd4) << 20;
+    const uint32_t jc = (b >> 16) & 7u;
+    uint32_t jt = 0;
+    if (jc >= 1 && jc <= 5) jt = d.regs[(uint64_t)nn * 3 + 1 + (jc > 2 ? 1 : 0)] + ((jc & 1u) == 0 ? 1u : 0u);
+    if (jc == 6) { if (ndv < d.F) jt |= d.fends[ndv] - 6u; else atomicOr(d.flags, EF_OOB); }
+    if (jc == 7) { if (ndv < d.F) jt |= d.fstarts[ndv]; else atomicOr(d.flags, EF_OOB); }
+    uint32_t rs[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        uint32_t oc = (b >> (7 + 3 * s)) & 7u;
+        uint32_t arg = 0;
+        if (oc == 1 || oc == 4 || oc == 5 || oc == 6) {
+            uint64_t ri = (uint64_t)nn * 3;
+            if (oc == 1) ri += s;
+            if (oc == 4 || oc == 5) ri += 1;
+            if (oc == 5) ri -= s;
+            arg = d.regs[ri];
+        }
+        if (oc == 2) arg |= ndv + 10u;
+        if (oc == 3) arg |= ndv + 42u;
+        if (oc == 7) arg |= iib + 63u;
+        rs[s] = arg;
+    }
+    L.loc = iloc; L.word = word; L.rd = rd; L.rs1 = rs[0]; L.rs2 = rs[1]; L.jt = jt;
+}
+
+// Parent-slot propagation: the last propagating lane of a node wins the slot.
+__global__ __launch_bounds__(256) void k_gen_regs(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    int32_t slot = -1; uint32_t val = 0;
+    for (uint32_t o = 0; o < 4; ++o) {
+        Lane L;
+        gen_lane(d, (uint32_t)i, o, false, L);
+        if (L.pai >= 0) { slot = L.pai; val = L.dpv; }
+    }
+    if (slot >= 0) {
+        if ((uint64_t)slot < d.n * 3) d.regs[slot] = val;
+        else atomicOr(d.flags, EF_OOB);
+    }
+}
+
+DEV uint64_t owner_key(const Dev &d, uint32_t nn, uint32_t o, bool use_rank) {
+    uint64_t level = (uint64_t)(d.maxd - (uint32_t)d.depth[nn]) + 1u;
+    uint64_t r = use_rank ? d.rank[nn] : nn;
+    return (level << 34) | (r << 2) | o;
+}
+
+__global__ __launch_bounds__(256) void k_gen_owner(Dev d, int use_rank) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    for (uint32_t o = 0; o < 4; ++o) {
+        Lane L;
+        gen_lane(d, (uint32_t)i, o, true, L);
+        if (!L.has) continue;
+        if (L.loc >= d.W0) { atomicOr(d.flags, EF_OOB); continue; }
+        atomicMax((unsigned long long *)&d.owner[L.loc], (unsigned long long)owner_key(d, (uint32_t)i, o, use_rank != 0));
+    }
+}
+
+// Counts slots won by one node but also written by another node of equal depth.
+__global__ __launch_bounds__(256) void k_gen_ties(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    for (uint32_t o = 0; o < 4; ++o) {
+        Lane L;
+        gen_lane(d, (uint32_t)i, o, true, L);
+        if (!L.has || L.loc >= d.W0) continue;
+        uint64_t k = owner_key(d, (uint32_t)i, o, false);
+        uint64_t w = d.owner[L.loc];
+        if ((w >> 34) == (k >> 34) && ((w >> 2) & 0xFFFFFFFFull) != i) atomicAdd(d.counter, 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gen_write(Dev d, int use_rank) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    for (uint32_t o = 0; o < 4; ++o) {
+        Lane L;
+        gen_lane(d, (uint32_t)i, o, true, L);
+        if (!L.has || L.loc >= d.W0) continue;
+        if (d.owner[L.loc] != owner_key(d, (uint32_t)i, o, use_rank != 0)) continue;
+        d.w0_instr[L.loc] = L.word; d.w0_rd[L.loc] = L.rd; d.w0_rs1[L.loc] = L.rs1;
+        d.w0_rs2[L.loc] = L.rs2; d.w0_jt[L.loc] = L.jt;
+    }
+}
+
+// ------------------------------------------ libstdc++ introsort, reproduced
+// std::sort over node ids with key depth (rv_generator_avx.cpp:539-541) is
+// introsort: depth limit 2*floor(log2 n), median of (first+1, mid, last-1)
+// moved to first, unguarded Hoare partition around it, ranges <= 16 left for a
+// final insertion sort, heapsort where the limit runs out.  The k-th left stop
+// (key >= pivot, scanning up from first+1) swaps with the k-th right stop
+// (key <= pivot, scanning down from last-1) while they have not crossed, so one
+// partition is two stream compactions and a search; all ranges of one
+// recursion depth are partitioned together.
+struct Seg { uint32_t first, last, limit, pad; };
+
+__global__ void k_sort_init(uint32_t *sidx, int32_t *skey, const int32_t *depth, uint64_t n, int32_t *seg_of) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sidx[i] = (uint32_t)i; skey[i] = depth[i]; seg_of[i] = 0;
+}
+
+DEV void swap_pos(uint32_t *sidx, int32_t *skey, uint32_t a, uint32_t b) {
+    uint32_t ti = sidx[a]; sidx[a] = sidx[b]; sidx[b] = ti;
+    int32_t tk = skey[a]; skey[a] = skey[b]; skey[b] = tk;
+}
+
+__global__ void k_sort_median(Seg *segs, uint32_t m, uint32_t *sidx, int32_t *skey, int32_t *pivot) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    uint32_t first = segs[s].first, last = segs[s].last;
+    uint32_t a = first + 1, b = first + (last - first) / 2, c = last - 1;
+    int32_t ka = skey[a], kb = skey[b], kc = skey[c];
+    uint32_t pick;
+    if (ka < kb) { if (kb < kc) pick = b; else if (ka < kc) pick = c; else pick = a; }
+    else if (ka < kc) pick = a;
+    else if (kb < kc) pick = c;
+    else pick = b;
+    swap_pos(sidx, skey, first, pick);
+    pivot[s] = skey[first];
+}
+
+__global__ void k_sort_flags(const int32_t *seg_of, const Seg *segs, const int32_t *pivot, const int32_t *skey,
+                             uint64_t n, uint8_t *fl, uint8_t *fr) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t s = seg_of[i];
+    uint8_t l = 0, r = 0;
+    if (s >= 0 && i > segs[s].first) {
+        int32_t p = pivot[s], k = skey[i];
+        l = !(k < p); r = !(p < k);
+    }
+    fl[i] = l; fr[i] = r;
+}
+
+struct FlagScanOp {
+    const uint8_t *f; uint32_t *out; uint32_t *pos;
+    DEV uint32_t value(uint64_t i) const { return f[i]; }
+    DEV void emit(uint64_t i, uint32_t excl, uint32_t v) const { out[i] = excl; if (v) pos[excl] = (uint32_t)i; }
+};
+
+struct SegCut { uint32_t baseL, cntL, baseR, cntR, m, cut; };
+
+DEV uint32_t scan_at(const uint32_t *sc, uint64_t i, uint64_t n, uint32_t total) { return i >= n ? total : sc[i]; }
+
+__global__ void k_sort_search(const Seg *segs, uint32_t m, const uint32_t *scL, const uint32_t *scR,
+                              const uint32_t *Lpos, const uint32_t *Rpos, uint64_t n, const uint32_t *totals,
+                              SegCut *cuts) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    uint32_t first = segs[s].first, last = segs[s].last;
+    uint32_t bL = scan_at(scL, first, n, totals[0]), eL = scan_at(scL, last, n, totals[0]);
+    uint32_t bR = scan_at(scR, first, n, totals[1]), eR = scan_at(scR, last, n, totals[1]);
+    uint32_t cL = eL - bL, cR = eR - bR;
+    // m = largest k with L_k < R_k (1-based), L_k = Lpos[bL+k-1], R_k = Rpos[bR+cR-k]
+    uint32_t lo = 0, hi = cL < cR ? cL : cR;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) / 2;
+        if (Lpos[bL + mid - 1] < Rpos[bR + cR - mid]) lo = mid; else hi = mid - 1;
+    }
+    uint32_t mm = lo;
+    uint32_t cut = 0xFFFFFFFFu;
+    if (mm + 1 <= cL) cut = Lpos[bL + mm];
+    if (mm >= 1) { uint32_t r = Rpos[bR + cR - mm]; if (r < cut) cut = r; }
+    if (cut <= first || cut >= last) cut = first + 1;  // unreachable after a median-of-three pivot
+    cuts[s] = SegCut{bL, cL, bR, cR, mm, cut};
+}
+
+__global__ void k_sort_swap(const int32_t *seg_of, const SegCut *cuts, const uint8_t *fl, const uint32_t *scL,
+                            const uint32_t *Rpos, uint64_t n, uint32_t *sidx, int32_t *skey) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n || !fl[i]) return;
+    int32_t s = seg_of[i];
+    SegCut c = cuts[s];
+    uint32_t k = scL[i] - c.baseL + 1;
+    if (k > c.m) return;
+    uint32_t partner = Rpos[c.baseR + c.cntR - k];
+    swap_pos(sidx, skey, (uint32_t)i, partner);
+}
+
+// Children of each partitioned range: next level, heapsort list, or leaves.
+__global__ void k_sort_split(const Seg *segs, const SegCut *cuts, uint32_t m, Seg *next, uint32_t *next_cnt,
+                             Seg *heap, uint32_t *heap_cnt, int32_t *child_ids) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    Seg g = segs[s];
+    uint32_t cut = cuts[s].cut;
+    uint32_t lim = g.limit - 1;
+    Seg ch[2] = {{g.first, cut, lim, 0}, {cut, g.last, lim, 0}};
+    for (int c = 0; c < 2; ++c) {
+        uint32_t sz = ch[c].last - ch[c].first;
+        int32_t id = -1;
+        if (sz > 16) {
+            if (lim == 0) { uint32_t h = atomicAdd(heap_cnt, 1u); heap[h] = ch[c]; }
+            else { uint32_t q = atomicAdd(next_cnt, 1u); next[q] = ch[c]; id = (int32_t)q; }
+        }
+        child_ids[2 * s + c] = id;
+    }
+}
+
+__global__ void k_sort_segof(int32_t *seg_of, const SegCut *cuts, const int32_t *child_ids, uint64_t n) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t s = seg_of[i];
+    if (s < 0) return;
+    seg_of[i] = child_ids[2 * s + (i >= cuts[s].cut ? 1 : 0)];
+}
+
+DEV bool KLESS(const int32_t *skey, uint32_t a, uint32_t b) { return skey[a] < skey[b]; }
+
+// libstdc++ heap routines on one range, one thread (std::__make_heap,
+// std::__adjust_heap, std::__push_heap, std::__sort_heap).
+DEV void heap_adjust(uint32_t *si, int32_t *sk, uint32_t base, int64_t hole, int64_t len, uint32_t vi, int32_t vk) {
+    const int64_t top = hole;
+    int64_t child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (sk[base + child] < sk[base + child - 1]) child--;
+        si[base + hole] = si[base + child]; sk[base + hole] = sk[base + child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        si[base + hole] = si[base + child - 1]; sk[base + hole] = sk[base + child - 1];
+        hole = child - 1;
+    }
+    int64_t parent = (hole - 1) / 2;
+    while (hole > top && sk[base + parent] < vk) {
+        si[base + hole] = si[base + parent]; sk[base + hole] = sk[base + parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    si[base + hole] = vi; sk[base + hole] = vk;
+}
+
+__global__ void k_sort_heap(const Seg *heap, const uint32_t *heap_cnt, uint32_t *si, int32_t *sk) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= *heap_cnt) return;
+    uint32_t base = heap[s].first;
+    int64_t len = heap[s].last - heap[s].first;
+    if (len >= 2) {
+        for (int64_t parent = (len - 2) / 2;; --parent) {
+            heap_adjust(si, sk, base, parent, len, si[base + parent], sk[base + parent]);
+            if (parent == 0) break;
+        }
+    }
+    for (int64_t last = len; last > 1;) {
+        --last;
+        uint32_t vi = si[base + last]; int32_t vk = sk[base + last];
+        si[base + last] = si[base]; sk[base + last] = sk[base];
+        heap_adjust(si, sk, base, 0, last, vi, vk);
+    }
+}
+
+// Leaves are the ranges between consecutive boundaries; boundaries are marked in
+// `bound` (1 at each range start).  Each boundary thread insertion-sorts its range.
+__global__ void k_sort_insertion(const uint8_t *bound, uint64_t n, uint32_t *si, int32_t *sk) {
+    uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (b >= n || !bound[b]) return;
+    uint64_t e = b + 1;
+    while (e < n && !bound[e]) ++e;
+    for (uint64_t i = b + 1; i < e; ++i) {
+        uint32_t vi = si[i]; int32_t vk = sk[i];
+        uint64_t j = i;
+        while (j > b && vk < sk[j - 1]) { si[j] = si[j - 1]; sk[j] = sk[j - 1]; --j; }
+        si[j] = vi; sk[j] = vk;
+    }
+}
+
+// Every cut starts a new range; leaf ranges are the spans between marks.
+__global__ void k_mark_cuts(const SegCut *cuts, uint32_t m, uint8_t *bound) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    bound[cuts[s].cut] = 1;
+}
+
+__global__ void k_sort_rank(const uint32_t *si, uint64_t n, uint32_t *rank) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rank[si[i]] = (uint32_t)i;
+}
+
+// ================================================================= optimize
+__global__ __launch_bounds__(256) void k_opt_init(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.W0) return;
+    int32_t a = (int32_t)(d.w0_rs1[i] - 64u), b = (int32_t)(d.w0_rs2[i] - 64u);
+    uint32_t w = d.w0_instr[i] & 0x707Fu;
+    bool st = w == 0x2023u || w == 0x2027u;
+    if (a >= 0) { if ((uint32_t)a < d.W0) { d.used[a] = 1; if (st) d.sstore[a] = 1; } else atomicOr(d.flags, EF_OOB); }
+    if (b >= 0) { if ((uint32_t)b < d.W0) { d.used[b] = 1; if (st) d.sstore[b] = 1; } else atomicOr(d.flags, EF_OOB); }
+}
+
+// One sweep: a word whose destination register is unused clears its sources
+// (except sources of stores).  Repeated until a sweep changes nothing.
+__global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.W0) return;
+    uint32_t rd = d.w0_rd[i];
+    if (!((int32_t)rd > 63)) return;
+    uint32_t v = rd - 64u;
+    if (v >= d.W0 || d.used[v]) return;
+    int32_t a = (int32_t)(d.w0_rs1[i] - 64u), b = (int32_t)(d.w0_rs2[i] - 64u);
+    bool ch = false;
+    if (a >= 0 && (uint32_t)a < d.W0 && !d.sstore[a] && d.used[a]) { d.used[a] = 0; ch = true; }
+    if (b >= 0 && (uint32_t)b < d.W0 && !d.sstore[b] && d.used[b]) { d.used[b] = 0; ch = true; }
+    if (ch) *changed = 1;
+}
+
+__global__ __launch_bounds__(256) void k_opt_final(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.W0) return;
+    d.usedi[i] = (!((int32_t)d.w0_rd[i] > 63) || d.used[i]) ? 1 : 0;
+}
+
+// ================================================================= regalloc
+struct RaStep {
+    uint32_t ord, rs1, rs2, r1r, r2r, rdr;
+    bool sw1, sw2, swd;
+    uint64_t newlife;
+};
+
+// One allocation step of one function (rv_generator_avx.cpp:1062-1397) with the
+// swapped states of its two operands given.
+DEV RaStep ra_eval(const Dev &d, uint32_t off, uint64_t life, bool s1, bool s2) {
+    RaStep r;
+    const uint32_t ins = d.w0_instr[off];
+    r.rs1 = d.w0_rs1[off]; r.rs2 = d.w0_rs2[off]; r.ord = d.w0_rd[off];
+    const bool v1 = (int32_t)r.rs1 > 63 && r.rs1 - 64u < d.W0, v2 = (int32_t)r.rs2 > 63 && r.rs2 - 64u < d.W0;
+    if (((int32_t)r.rs1 > 63 && !v1) || ((int32_t)r.rs2 > 63 && !v2)) atomicOr(d.flags, EF_OOB);
+    r.r1r = v1 ? d.sym_reg[r.rs1 - 64u] : 0u;
+    r.r2r = v2 ? d.sym_reg[r.rs2 - 64u] : 0u;
+    if (v1 && s1) r.r1r = (ins == 0xC0007053u || (ins & 0x7Fu) == 0x53u) ? 37u : 5u;
+    if (v2 && s2) r.r2r = ((ins & 0x7Fu) == 0x53u) ? 38u : 6u;
+    const uint64_t clr = ((1ull << r.r1r) | (1ull << r.r2r)) & ~1ull;
+    const uint64_t cl = life & ~clr;
+    uint32_t rdr = r.ord;
+    if ((int32_t)r.ord > 63) {
+        const bool rdf = ins == 0xD0007053u || (ins & 0x7Fu) == 0x53u;
+        rdr = ffz_gcc(cl | (rdf ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull));
+        if (rdr == 64) rdr = rdf ? 37u : 5u;
+    }
+    r.rdr = rdr;
+    const uint64_t rdm = 1ull << rdr;
+    r.sw1 = r.r1r != 0 && ((life >> r.r1r) & 1ull);
+    r.sw2 = r.r2r != 0 && ((life >> r.r2r) & 1ull);
+    r.swd = rdr != 0 && ((cl >> rdr) & 1ull);
+    r.newlife = cl | rdm;
+    return r;
+}
+
+DEV void ra_commit_syms(const Dev &d, const RaStep &r, bool reset_sw) {
+    if (((int32_t)r.ord > 63 && r.ord - 64u >= d.W0) || ((int32_t)r.rs1 > 63 && r.rs1 - 64u >= d.W0) ||
+        ((int32_t)r.rs2 > 63 && r.rs2 - 64u >= d.W0)) { atomicOr(d.flags, EF_OOB); return; }
+    if ((int32_t)r.ord > 63) { d.sym_reg[r.ord - 64u] = (uint8_t)r.rdr; if (reset_sw) d.sym_sw[r.ord - 64u] = 0; }
+    if ((int32_t)r.rs1 > 63) { d.sym_reg[r.rs1 - 64u] = (uint8_t)r.r1r; if (reset_sw) d.sym_sw[r.rs1 - 64u] = 0; }
+    if ((int32_t)r.rs2 > 63) { d.sym_reg[r.rs2 - 64u] = (uint8_t)r.r2r; if (reset_sw) d.sym_sw[r.rs2 - 64u] = 0; }
+}
+
+// Functions outside lane 0 of their group: their swapped flags are never set
+// (only lane-0 registers are ever looked up), so each walks alone.  Per step,
+// the registers each group looks up are recorded as min/max group per register.
+__global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin, uint32_t *gmax) {
+    __shared__ uint32_t smin[64], smax[64];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t t = blockIdx.x * 64u + lane;
+    const uint32_t f = 8u * (t / 7u) + 1u + (t % 7u);
+    const bool act = f < d.F;
+    const uint32_t start = act ? d.fstarts[f] : 0u;
+    const uint32_t size = act ? d.fsizes[f] : 0u;
+    const uint32_t grp = f >> 3;
+    smin[lane] = 0xFFFFFFFFu; smax[lane] = 0;
+    uint32_t wmax = size;
+    for (int s = 32; s > 0; s >>= 1) { uint32_t o = __shfl_xor(wmax, s, 64); wmax = wmax > o ? wmax : o; }
+    uint64_t life = 0x1Full;
+    __syncthreads();
+    for (uint32_t i = 0; i < wmax; ++i) {
+        if ((int32_t)i < (int32_t)size) {
+            const uint32_t off = start + i;
+            RaStep r = ra_eval(d, off, life, false, false);
+            if (r.swd) { atomicMin(&smin[r.rdr], grp); atomicMax(&smax[r.rdr], grp + 1u); }
+            if (r.sw1) { atomicMin(&smin[r.r1r], grp); atomicMax(&smax[r.r1r], grp + 1u); }
+            if (r.sw2) { atomicMin(&smin[r.r2r], grp); atomicMax(&smax[r.r2r], grp + 1u); }
+            life = r.newlife;
+            ra_commit_syms(d, r, false);
+        }
+        __syncthreads();
+        uint32_t mx = smax[lane];
+        if (mx) {
+            atomicMin(&gmin[(uint64_t)i * 64u + lane], smin[lane]);
+            atomicMax(&gmax[(uint64_t)i * 64u + lane], mx);
+            smin[lane] = 0xFFFFFFFFu; smax[lane] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// Lane-0 functions in step lockstep, one workgroup.  Region 0 of the
+// reference's register_state holds the last lane-0 write per physical register;
+// every lookup of any function at step i reads its step-start snapshot S and
+// marks the virtual register found there as swapped.  A lookup by a lower group
+// at step i is seen by the owner's operand read at step i; one by a higher group
+// lands after the owner's own reset.  Intra-step effects between lane-0
+// functions are settled by re-evaluation until stable (lower groups settle
+// first, so this converges to the reference's in-order result).
+constexpr int RA0_THREADS = 1024;
+
+struct Ra0Rec { uint32_t ord, rs1, rs2; uint32_t regs; /* r1r | r2r<<8 | rdr<<16 */ };
+
+DEV void ra0_flips(const int32_t *So, const uint32_t *fbits, uint32_t j, bool &f1, bool &f2) {
+    f1 = false; f2 = false;
+#pragma unroll 8
+    for (int r = 0; r < 64; ++r)
+        if (So[r] == (int32_t)j) { f1 |= (fbits[r] & 1u) != 0; f2 |= (fbits[r] & 2u) != 0; }
+}
+
+DEV RaStep ra0_eval(const Dev &d, uint32_t off, uint64_t life, bool f1, bool f2) {
+    const uint32_t rs1 = d.w0_rs1[off], rs2 = d.w0_rs2[off];
+    bool s1 = ((int32_t)rs1 > 63 && rs1 - 64u < d.W0 && d.sym_sw[rs1 - 64u]) || f1;
+    bool s2 = ((int32_t)rs2 > 63 && rs2 - 64u < d.W0 && d.sym_sw[rs2 - 64u]) || f2;
+    return ra_eval(d, off, life, s1, s2);
+}
+
+__global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0(Dev d, uint32_t maxsize, const uint32_t *gmin,
+                                                         const uint32_t *gmax, uint64_t *life0, Ra0Rec *rec) {
+    __shared__ int32_t Rv[64], Ro[64], Sv[64], So[64];
+    __shared__ uint32_t Lmin[64], Lmax[64], fbits[64], wkey[64];
+    __shared__ int changed;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t L = (d.F + 7u) / 8u;
+    if (tid < 64) { Rv[tid] = -1; Ro[tid] = -1; }
+    for (uint32_t j = tid; j < L; j += RA0_THREADS) life0[j] = 0x1Full;
+    __syncthreads();
+    for (uint32_t i = 0; i < maxsize; ++i) {
+        if (tid < 64) { Sv[tid] = Rv[tid]; So[tid] = Ro[tid]; fbits[tid] = 0; wkey[tid] = 0; }
+        __syncthreads();
+        for (uint32_t round = 0;; ++round) {
+            if (tid < 64) { Lmin[tid] = gmin[(uint64_t)i * 64u + tid]; Lmax[tid] = gmax[(uint64_t)i * 64u + tid]; }
+            if (tid == 0) changed = 0;
+            __syncthreads();
+            for (uint32_t j = tid; j < L; j += RA0_THREADS) {
+                const uint32_t f = 8u * j;
+                if (!((int32_t)i < (int32_t)d.fsizes[f])) continue;
+                bool f1, f2;
+                ra0_flips(So, fbits, j, f1, f2);
+                RaStep r = ra0_eval(d, d.fstarts[f] + i, life0[j], f1, f2);
+                if (r.swd) { atomicMin(&Lmin[r.rdr], j); atomicMax(&Lmax[r.rdr], j + 1u); }
+                if (r.sw1) { atomicMin(&Lmin[r.r1r], j); atomicMax(&Lmax[r.r1r], j + 1u); }
+                if (r.sw2) { atomicMin(&Lmin[r.r2r], j); atomicMax(&Lmax[r.r2r], j + 1u); }
+            }
+            __syncthreads();
+            if (tid < 64) {
+                uint32_t nb = 0;
+                if (Sv[tid] >= 64 && So[tid] >= 0) {
+                    const uint32_t o = (uint32_t)So[tid];
+                    const uint32_t f = 8u * o;
+                    if ((int32_t)i < (int32_t)d.fsizes[f] && Lmin[tid] < o) {
+                        const uint32_t off = d.fstarts[f] + i;
+                        if (d.w0_rs1[off] == (uint32_t)Sv[tid]) nb |= 1u;
+                        if (d.w0_rs2[off] == (uint32_t)Sv[tid]) nb |= 2u;
+                    }
+                }
+                if (nb != fbits[tid]) changed = 1;
+                Lmin[tid] = nb;  // staged; fbits is updated after the barrier
+            }
+            __syncthreads();
+            int again = changed;
+            if (again && round > L + 1) { again = 0; if (tid == 0) atomicOr(d.flags, EF_OOB); }
+            if (tid < 64 && again) fbits[tid] = Lmin[tid];
+            __syncthreads();
+            if (!again) break;
+        }
+        // commit: lifetime, records, region-0 write keys (group order: last writer wins)
+        for (uint32_t j = tid; j < L; j += RA0_THREADS) {
+            const uint32_t f = 8u * j;
+            if (!((int32_t)i < (int32_t)d.fsizes[f])) continue;
+            bool f1, f2;
+            ra0_flips(So, fbits, j, f1, f2);
+            RaStep r = ra0_eval(d, d.fstarts[f] + i, life0[j], f1, f2);
+            life0[j] = r.newlife;
+            rec[j] = Ra0Rec{r.ord, r.rs1, r.rs2, r.r1r | (r.r2r << 8) | (r.rdr << 16)};
+            if (r.ord) atomicMax(&wkey[r.rdr], ((j + 1u) << 2) | 1u);
+            if (r.r1r != r.rdr) atomicMax(&wkey[r.r1r], ((j + 1u) << 2) | 2u);
+            if (r.r2r != r.rdr) atomicMax(&wkey[r.r2r], ((j + 1u) << 2) | 3u);
+        }
+        __syncthreads();
+        for (uint32_t j = tid; j < L; j += RA0_THREADS) {
+            const uint32_t f = 8u * j;
+            if (!((int32_t)i < (int32_t)d.fsizes[f])) continue;
+            const Ra0Rec q = rec[j];
+            RaStep r;
+            r.ord = q.ord; r.rs1 = q.rs1; r.rs2 = q.rs2;
+            r.r1r = q.regs & 0xFFu; r.r2r = (q.regs >> 8) & 0xFFu; r.rdr = (q.regs >> 16) & 0xFFu;
+            if (r.ord && wkey[r.rdr] == (((j + 1u) << 2) | 1u)) { Rv[r.rdr] = (int32_t)r.ord; Ro[r.rdr] = (int32_t)j; }
+            if (r.r1r != r.rdr && wkey[r.r1r] == (((j + 1u) << 2) | 2u)) { Rv[r.r1r] = -1; Ro[r.r1r] = -1; }
+            if (r.r2r != r.rdr && wkey[r.r2r] == (((j + 1u) << 2) | 3u)) { Rv[r.r2r] = -1; Ro[r.r2r] = -1; }
+            ra_commit_syms(d, r, true);
+        }
+        __syncthreads();
+        // marks on the snapshot's virtual registers
+        if (tid < 64 && Sv[tid] >= 64 && Lmax[tid] != 0) {
+            const uint32_t v = (uint32_t)Sv[tid];
+            bool mark = true;
+            if (So[tid] >= 0) {
+                const uint32_t o = (uint32_t)So[tid];
+                const uint32_t f = 8u * o;
+                if ((int32_t)i < (int32_t)d.fsizes[f]) {
+                    const Ra0Rec q = rec[o];
+                    bool reset = (q.ord == v) || (q.rs1 == v) || (q.rs2 == v);
+                    if (reset) mark = Lmax[tid] > o + 1u;
+                }
+            }
+            if (mark) {
+                if (v - 64u < d.W0) d.sym_sw[v - 64u] = 1;
+                else atomicOr(d.flags, EF_OOB);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Instruction counts after allocation (:1473-1495): used words plus one load
+// per swapped operand and one store for a swapped destination.
+struct RaCountOp {
+    Dev d;
+    DEV uint32_t sw(uint32_t r) const { return ((int32_t)r > 63 && r - 64u < d.W0 && d.sym_sw[r - 64u]) ? 1u : 0u; }
+    DEV uint32_t value(uint64_t i) const {
+        if (!d.usedi[i]) return 0;
+        return 1u + sw(d.w0_rs1[i]) + sw(d.w0_rs2[i]) + sw(d.w0_rd[i]);
+    }
+    DEV void emit(uint64_t i, uint32_t excl, uint32_t) const { d.ra_off[i] = excl; }
+};
+
+// spill_offsets (:1450-1469): each function's words copy the flag at its start.
+DEV uint32_t spill_of(const Dev &d, uint32_t v) {
+    uint32_t lo = 0, hi = d.F;  // last start <= v
+    while (hi - lo > 1) { uint32_t m = (lo + hi) / 2; if (d.fstarts[m] <= v) lo = m; else hi = m; }
+    return d.sym_sw[d.fstarts[lo]] ? 1u : 0u;
+}
+
+// Emit (:1552-1697).
+__global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.W0 || !d.usedi[i]) return;
+    const int32_t base = (int32_t)d.ra_off[i];
+    const uint32_t ins = d.w0_instr[i];
+    const uint32_t r1 = d.w0_rs1[i], r2 = d.w0_rs2[i], rdv = d.w0_rd[i];
+    int32_t l1 = -1, l2 = -1, rdo = -1;
+    uint32_t a1 = r1, a2 = r2, ad = rdv, st1 = 0, st2 = 0, std_ = 0;
+    if (((int32_t)r1 > 63 && r1 - 64u >= d.W0) || ((int32_t)r2 > 63 && r2 - 64u >= d.W0) ||
+        ((int32_t)rdv > 63 && rdv - 64u >= d.W0)) { atomicOr(d.flags, EF_OOB); return; }
+    if ((int32_t)r1 > 63) {
+        bool sw = d.sym_sw[r1 - 64u];
+        l1 = sw ? base : -1;
+        st1 = spill_of(d, r1 - 64u);
+        a1 = sw ? (((ins == 0xC0007053u) || (ins & 0x7Fu) == 0x53u) ? 37u : 5u) : d.sym_reg[r1 - 64u];
+    }
+    if ((int32_t)r2 > 63) {
+        bool sw = d.sym_sw[r2 - 64u];
+        l2 = sw ? base : -1;  // the reference tests its own, still -1, offset (D6)
+        st2 = spill_of(d, r2 - 64u);
+        a2 = sw ? (((ins & 0x7Fu) == 0x53u) ? 38u : 6u) : d.sym_reg[r2 - 64u];
+    }
+    const int32_t mo = base + (l1 > 0 ? 1 : 0) + (l2 > 0 ? 1 : 0);
+    if ((int32_t)rdv > 63) {
+        bool sw = d.sym_sw[rdv - 64u];
+        rdo = sw ? mo + 1 : -1;
+        std_ = spill_of(d, rdv - 64u);
+        ad = d.sym_reg[rdv - 64u];
+    }
+    if ((rdo >= 0 && (uint32_t)rdo >= d.W1) || (uint32_t)mo >= d.W1) { atomicOr(d.flags, EF_OOB); return; }
+    if (l1 >= 0) { d.w1_instr[l1] = (((st1 - 1u) * 4u) << 20) | 0x2003u; d.w1_rd[l1] = 5; }
+    if (l2 >= 0) { d.w1_instr[l2] = (((st2 - 1u) * 4u) << 20) | 0x2003u; d.w1_rd[l2] = 6; }
+    if (mo >= 0) {
+        d.w1_instr[mo] = ins; d.w1_rd[mo] = ad; d.w1_rs1[mo] = a1; d.w1_rs2[mo] = a2;
+        d.w1_jt[mo] = d.ra_off[d.w0_jt[i] < d.W0 ? d.w0_jt[i] : 0];
+        if (d.w0_jt[i] >= d.W0) atomicOr(d.flags, EF_OOB);
+    }
+    if (rdo >= 0) {
+        uint32_t imm = (std_ - 1u) * (uint32_t)-4;
+        d.w1_instr[rdo] = 0x2023u | ((imm & 0x1Fu) << 7) | ((imm & 0xFEu) << 19);
+        d.w1_rs1[rdo] = ad;
+    }
+}
+
+// fix_func_tab (:1816-1837) against an offset array of `len` entries.
+__global__ void k_fixtab(Dev d, const uint32_t *off, uint32_t len) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.F) return;
+    uint32_t s = d.fstarts[k], e = s + d.fsizes[k];
+    uint32_t so = off[s];
+    uint32_t eo = (int32_t)e >= (int32_t)len ? off[len - 1] + 1u : off[e];
+    d.fstarts[k] = so; d.fends[k] = eo; d.fsizes[k] = eo - so;
+}
+
+// Frame words (:1714-1811): lui s0, 0 / addi s0, s0, 8 after the prologue
+// stores and before the epilogue; no callee-saved preservation (never written).
+__global__ void k_ra_frame(Dev d) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.F) return;
+    const uint32_t upper = 0x437u, lower = (8u << 20) | 0x40413u;
+    uint32_t s = d.fstarts[k], e = s + d.fsizes[k];
+    uint32_t pos[4] = {s + 2, s + 3, e - 6, e - 5};
+    for (int q = 0; q < 4; ++q) {
+        uint32_t x = pos[q];
+        if (x >= d.W1) { atomicOr(d.flags, EF_OOB); continue; }
+        d.w1_instr[x] = (q & 1) ? lower : upper;
+        d.w1_rd[x] = 0; d.w1_rs1[x] = 0; d.w1_rs2[x] = 0; d.w1_jt[x] = 0;
+    }
+}
+
+// ================================================================ fix_jumps
+DEV uint32_t fj_size(uint32_t w) {
+    bool jmp = ((w & 0x7Fu) == 0x67u) && w != 0x8067u;
+    return jmp ? 2u : (w != 0 ? 1u : 0u);
+}
+
+struct FjSizeOp {
+    Dev d;
+    DEV uint32_t value(uint64_t i) const { return fj_size(d.w1_instr[i]); }
+    DEV void emit(uint64_t i, uint32_t excl, uint32_t) const { d.fj_off[i] = excl; }
+};
+
+// Move + jump/branch encoding + postprocess in one pass (:1895-1993, :1995-2011).
+__global__ __launch_bounds__(256) void k_fj_move(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.W1) return;
+    const uint32_t w = d.w1_instr[i];
+    if (w == 0) return;
+    const uint32_t k = d.fj_off[i];
+    const bool jmp = ((w & 0x7Fu) == 0x67u) && w != 0x8067u;
+    const bool br = (w & 0x7Fu) == 0x63u;
+    const uint32_t pp = ((d.w1_rd[i] & 31u) << 7) | ((d.w1_rs1[i] & 31u) << 15) | ((d.w1_rs2[i] & 31u) << 20);
+    if (jmp || br) {
+        const uint32_t jt1 = d.w1_jt[i];
+        if (jt1 >= d.W1) atomicOr(d.flags, EF_OOB);
+        const uint32_t tgt = 4u * d.fj_off[jt1 < d.W1 ? jt1 : 0];
+        const uint32_t delta = tgt - k * 4u;
+        if (jmp) {
+            const uint32_t upper = (delta & 0xFFFFF000u) >> 12, lower = delta & 0xFFFu, sign = (delta >> 11) & 1u;
+            d.words[k] = ((upper + sign) << 12) | 0x97u;
+            d.words[k + 1] = w | (lower << 20) | (1u << 15);
+            return;
+        }
+        const uint32_t sign = (delta >> 12) & 1u, b11 = (delta >> 11) & 1u;
+        const uint32_t b105 = (delta >> 5) & 0x1Fu, b14 = (delta >> 1) & 0xFu;
+        d.words[k] = w | (sign << 31) | (b11 << 7) | (b105 << 25) | (b14 << 8) | pp;
+        return;
+    }
+    d.words[k] = w | pp;
+}
+
+// ================================================================ host side
+#include "ptl_host.inc"

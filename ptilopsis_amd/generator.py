@@ -1,0 +1,141 @@
+"""Host-side mirror of the reference's rv_generator interface over the HIP library.
+
+The reference's interface (include/codegen/rv_generator.hpp:19-74):
+``rv_generator(const DepthTree&)``, ``process(bool profile)``,
+``get_instructions()``, ``to_binary``/``to_asm``/``print``, and the protected
+function table.  :class:`RvGeneratorHip` keeps those names and their meaning;
+errors are raised as ``RuntimeError(ptl_strerror(status))`` the way the
+reference throws ``std::runtime_error``.  Every call goes through
+``libptl_hip.so``; there is no other code path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+
+import numpy as np
+
+from . import abi
+from .native import Tree, hip_lib
+
+
+class PtlError(RuntimeError):
+    def __init__(self, status: int):
+        self.status = status
+        super().__init__(hip_lib().ptl_strerror(status).decode())
+
+
+class Context:
+    """One ptl_ctx: a HIP stream plus a pooled device arena on one device."""
+
+    def __init__(self, device: int = 0, arena_bytes: int = 0):
+        self.lib = hip_lib()
+        h = C.c_void_p()
+        s = self.lib.ptl_create(C.byref(h), device, arena_bytes)
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ptl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _ast(tree: Tree):
+        keep = [np.ascontiguousarray(a) for a in tree.arrays()]
+        a = abi.PtlAst(*[k.ctypes.data for k in keep], tree.n, tree.max_depth)
+        return a, keep
+
+    def compile(self, tree: Tree) -> dict:
+        a, keep = self._ast(tree)
+        r = abi.PtlResult()
+        s = self.lib.ptl_compile(self.h, C.byref(a), C.byref(r))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return result_dict(r)
+
+    def upload(self, slot: int, tree: Tree) -> None:
+        a, keep = self._ast(tree)
+        s = self.lib.ptl_upload(self.h, slot, C.byref(a))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+
+    def compile_resident(self, slot: int, copy_out: bool = True) -> dict:
+        r = abi.PtlResult()
+        s = self.lib.ptl_compile_resident(self.h, slot, 1 if copy_out else 0, C.byref(r))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return result_dict(r, copy_out)
+
+    def stop_after(self, stage: int) -> None:
+        self.lib.ptl_debug_stop_after(self.h, stage)
+
+    def dump(self, name: str) -> np.ndarray:
+        n = self.lib.ptl_debug_dump(self.h, name.encode(), None, 0)
+        if n < 0:
+            raise PtlError(int(n))
+        out = np.zeros(max(n, 1), np.uint32)
+        self.lib.ptl_debug_dump(self.h, name.encode(), out.ctypes.data, n)
+        return out[:n]
+
+
+def result_dict(r: abi.PtlResult, copy_out: bool = True) -> dict:
+    def arr(p, n):
+        if not copy_out or n == 0:
+            return np.zeros(0, np.uint32)
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+    nf = r.n_funcs
+    return dict(words=arr(r.words, r.n_words), func_ids=arr(r.func_ids, nf), func_starts=arr(r.func_starts, nf),
+                func_ends=arr(r.func_ends, nf), func_sizes=arr(r.func_sizes, nf), n_words=int(r.n_words),
+                n_funcs=int(nf), stage_ns=[int(x) for x in r.stage_ns], W0=int(r.n_words_gen),
+                W1=int(r.n_words_alloc), opt_iterations=int(r.opt_iterations), tie_slots=int(r.tie_slots))
+
+
+class RvGeneratorHip:
+    """Drop-in for rv_generator_avx (rv_generator.hpp:19-74; rv_generator_avx.cpp:104-120)."""
+
+    def __init__(self, tree: Tree, ctx: Context | None = None):
+        self.tree = tree
+        self.ctx = ctx or Context()
+        self._res = None
+
+    def process(self, profile: bool = False) -> None:
+        """Runs the seven stages; prints the stage table to stderr and, with
+        profile, the two-line CSV to stdout (src/codegen/rv_generator.cpp:150-202)."""
+        self._res = self.ctx.compile(self.tree)
+        ns = self._res["stage_ns"]
+        if profile:
+            print(",".join(abi.STAGES) + "," + abi.STAGES[-1])
+            print(",".join(str(x) for x in ns) + "," + str(ns[-1]))
+        total = max(1, sum(ns))
+        w = max(len(s) for s in abi.STAGES) + 1
+        print(f"{'Stage':<{w}} {'Duration':>12} % of total", file=sys.stderr)
+        for name, t in zip(abi.STAGES, ns):
+            print(f"{name:<{w}} {t / 1e6:>9.3f} ms {100.0 * t / total:>9.2f}%", file=sys.stderr)
+        print(f"{'Total':<{w}} {total / 1e6:>9.3f} ms", file=sys.stderr)
+
+    def get_instructions(self) -> np.ndarray:
+        return self._res["words"]
+
+    @property
+    def function_ids(self): return self._res["func_ids"]
+
+    @property
+    def func_starts(self): return self._res["func_starts"]
+
+    @property
+    def func_ends(self): return self._res["func_ends"]
+
+    @property
+    def function_sizes(self): return self._res["func_sizes"]
+
+    def to_binary(self, stream) -> None:
+        """Raw little-endian words (rv_generator.cpp:116-119)."""
+        stream.write(self.get_instructions().astype("<u4").tobytes())

@@ -1,0 +1,158 @@
+"""ctypes bindings for the two in-tree shared libraries.
+
+``libptl_host.so``  host front end (source -> DepthTree arrays) and the seeded
+                    program generator; plain C++.
+``libptl_hip.so``   the code generator itself: HIP kernels for gfx950 behind the
+                    extern "C" interface of ``include/ptl.h``.
+
+Both are built in-tree by ``python -m ptilopsis_amd.build`` (or
+``__graft_entry__.build()``).  Loading fails loudly when a library is missing:
+there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_SO = os.path.join(HERE, "libptl_host.so")
+HIP_SO = os.path.join(HERE, "libptl_hip.so")
+
+_host = None
+_hip = None
+
+
+class HostTree(C.Structure):
+    _fields_ = [
+        ("node_types", C.POINTER(C.c_uint8)),
+        ("result_types", C.POINTER(C.c_uint8)),
+        ("parents", C.POINTER(C.c_int32)),
+        ("depth", C.POINTER(C.c_int32)),
+        ("child_idx", C.POINTER(C.c_int32)),
+        ("node_data", C.POINTER(C.c_uint32)),
+        ("n_nodes", C.c_uint64),
+        ("max_depth", C.c_uint32),
+        ("n_functions", C.c_uint32),
+    ]
+
+
+class GenParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("n_functions", C.c_uint32),
+        ("min_args", C.c_uint32), ("max_args", C.c_uint32),
+        ("min_stmts", C.c_uint32), ("max_stmts", C.c_uint32),
+        ("nested_min", C.c_uint32), ("nested_max", C.c_uint32),
+        ("max_nest", C.c_uint32),
+        ("p_assign", C.c_double), ("p_if", C.c_double), ("p_while", C.c_double), ("p_return", C.c_double),
+        ("p_var_leaf", C.c_double),
+        ("expr_p", C.c_double),
+        ("p_leaf_operand", C.c_double),
+        ("p_arith", C.c_double), ("p_compare", C.c_double),
+        ("float_frac", C.c_double),
+        ("balanced", C.c_uint32),
+        ("balanced_min", C.c_uint32), ("balanced_max", C.c_uint32),
+    ]
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_SO):
+            raise RuntimeError(f"{HOST_SO} is not built; run python -m ptilopsis_amd.build")
+        lib = C.CDLL(HOST_SO)
+        lib.ptl_parse_source.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(HostTree), C.c_char_p, C.c_uint64]
+        lib.ptl_parse_source.restype = C.c_int
+        lib.ptl_free_host_tree.argtypes = [C.POINTER(HostTree)]
+        lib.ptl_generate_source.argtypes = [C.POINTER(GenParams), C.POINTER(C.c_uint32), C.POINTER(C.c_char_p),
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        lib.ptl_generate_source.restype = C.c_int
+        lib.ptl_free_source.argtypes = [C.c_void_p]
+        _host = lib
+    return _host
+
+
+def hip_lib():
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_SO):
+            raise RuntimeError(f"{HIP_SO} is not built; run python -m ptilopsis_amd.build")
+        from . import abi
+        _hip = abi.bind(C.CDLL(HIP_SO))
+    return _hip
+
+
+@dataclass
+class Tree:
+    """The six DepthTree arrays (pareas-ast include/codegen/depthtree.hpp:12-58)."""
+    node_types: np.ndarray   # u8, pareas NodeType
+    result_types: np.ndarray  # u8, pareas DataType
+    parents: np.ndarray      # i32, root -1
+    depth: np.ndarray        # i32
+    child_idx: np.ndarray    # i32, root -1
+    node_data: np.ndarray    # u32
+    max_depth: int
+    n_functions: int = -1
+
+    @property
+    def n(self) -> int:
+        return int(self.node_types.shape[0])
+
+    def arrays(self):
+        return (self.node_types, self.result_types, self.parents, self.depth, self.child_idx, self.node_data)
+
+    def save(self, path: str) -> None:
+        np.savez(path, node_types=self.node_types, result_types=self.result_types, parents=self.parents,
+                 depth=self.depth, child_idx=self.child_idx, node_data=self.node_data,
+                 max_depth=np.array([self.max_depth], np.int64), n_functions=np.array([self.n_functions], np.int64))
+
+    @staticmethod
+    def load(path: str) -> "Tree":
+        z = np.load(path)
+        return Tree(z["node_types"], z["result_types"], z["parents"], z["depth"], z["child_idx"], z["node_data"],
+                    int(z["max_depth"][0]), int(z["n_functions"][0]))
+
+
+def parse_source(src: str | bytes) -> Tree:
+    """Front end: Pareas-language source -> DepthTree arrays (raises ValueError on parse errors)."""
+    lib = host_lib()
+    b = src.encode() if isinstance(src, str) else src
+    t = HostTree()
+    err = C.create_string_buffer(512)
+    if lib.ptl_parse_source(b, len(b), C.byref(t), err, 512) != 0:
+        raise ValueError("parse error: " + err.value.decode(errors="replace"))
+    n = t.n_nodes
+    try:
+        def arr(ptr, dt):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True) if n else np.zeros(0, dt)
+        tree = Tree(arr(t.node_types, np.uint8), arr(t.result_types, np.uint8), arr(t.parents, np.int32),
+                    arr(t.depth, np.int32), arr(t.child_idx, np.int32), arr(t.node_data, np.uint32),
+                    int(t.max_depth), int(t.n_functions))
+    finally:
+        lib.ptl_free_host_tree(C.byref(t))
+    return tree
+
+
+def generate_source(params: dict, targets=None) -> tuple[str, int]:
+    """Runs the seeded program generator; returns (source, expected node count)."""
+    lib = host_lib()
+    p = GenParams()
+    for k, v in params.items():
+        setattr(p, k, v)
+    out = C.c_char_p()
+    ln = C.c_uint64()
+    nodes = C.c_uint64()
+    tp = None
+    if targets is not None:
+        ta = np.ascontiguousarray(targets, dtype=np.uint32)
+        tp = ta.ctypes.data_as(C.POINTER(C.c_uint32))
+    if lib.ptl_generate_source(C.byref(p), tp, C.byref(out), C.byref(ln), C.byref(nodes)) != 0:
+        raise MemoryError("generator failed")
+    try:
+        s = C.string_at(out, ln.value).decode()
+    finally:
+        lib.ptl_free_source(out)
+    return s, int(nodes.value)
