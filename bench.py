@@ -1,0 +1,154 @@
+"""Benchmark: AST nodes/s to bit-exact RISC-V words on one MI355X.
+
+One step = one full pass of the code generator (the seven stages of
+rv_generator::process, src/codegen/rv_generator.cpp:129-148) over the
+large_16m workload (one synthetic Pareas program, ~16M AST nodes, 20k
+functions), with the DepthTree arrays resident in HBM and the words left in
+HBM.  `value` = nodes * steps / wall time of the timed steps.
+
+The JSON line also carries:
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                HIP-event duration on the library's stream, against 8 TB/s;
+  cpu_baseline  the CPU oracle (a one-thread restatement of the reference
+                generator) timed on the same input on this host, whose output
+                also checks the GPU result word for word (bit_exact);
+  stages_ms     per-stage device time.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large_16m]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "AST nodes/s to bit-exact RV64, batched vs plain GPU vs AVX2 host; p50/p99 lat"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_alg_bytes(name: str, n: int, w0: int, w1: int, w2: int, f: int) -> float | None:
+    """Algorithmic (compulsory) HBM bytes of one launch of a kernel: each input
+    element read once, each output element written once (u8 types, u32 else)."""
+    table = {
+        "k_pre12": 18 * n + 6 * n,          # read the six input arrays, write types, rtypes, node data
+        "k_pre3": 1 * n,                    # read input data types
+        "k_scan_count": 11 * n + 4 * n,     # read types, rtypes, parents, child idx; write locations
+        "k_cnt_fix": 14 * n,                # read parents, types, child idx, rtypes, locations
+        "k_scan_decl": 1 * n + 4 * f,
+        "k_gen_regs": 22 * n + 4 * n,       # node arrays + locations; parent slots
+        "k_gen_owner": 26 * n + 8 * w0,     # node arrays + slots; owner keys
+        "k_gen_write": 26 * n + 8 * w0 + 20 * w0,
+        "k_gen_ties": 26 * n + 8 * w0,
+        "k_opt_init": 16 * w0 + 2 * w0,
+        "k_opt_iter": 12 * w0 + 1 * w0,
+        "k_opt_final": 4 * w0 + 2 * w0,
+        "k_ra_walk": 16 * w0 * 7 // 8 + 2 * w0 * 7 // 8,
+        "k_ra_lane0": 16 * w0 // 8 + 2 * w0 // 8,
+        "k_scan_ra_count": 13 * w0 + 4 * w0,
+        "k_ra_emit": 24 * w0 + 2 * w0 + 20 * w1,
+        "k_scan_fj_size": 4 * w1 + 4 * w1,
+        "k_fj_move": 24 * w1 + 4 * w2,
+    }
+    return table.get(name)
+
+
+def build_workload(name: str):
+    from ptilopsis_amd import workloads as W
+    if name == "large_16m":
+        return W.large_16m()
+    if name == "single_50k":
+        return W.single_50k()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="large_16m")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: the code generator has no multi-GPU path "
+              "(one program per device; see DESIGN.md for how it would shard)", file=sys.stderr)
+        return 2
+
+    from ptilopsis_amd.generator import Context
+
+    t0 = time.time()
+    tree = build_workload(args.workload)
+    log(f"workload {args.workload}: {tree.n} nodes, {tree.n_functions} functions, depth {tree.max_depth} "
+        f"(generated+parsed in {time.time() - t0:.1f}s)")
+    ctx = Context(0)
+    ctx.upload(0, tree)
+    for i in range(args.warmup):
+        r = ctx.compile_resident(0, copy_out=False)
+        log(f"warmup {i}: {sum(r['stage_ns']) / 1e6:.1f} ms device")
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        r = ctx.compile_resident(0, copy_out=False)
+    t2 = time.perf_counter()
+    elapsed = t2 - t1
+    value = tree.n * args.steps / elapsed
+    log(f"timed {args.steps} steps: {elapsed * 1e3 / args.steps:.2f} ms/step, {value / 1e6:.1f} M nodes/s")
+
+    # one more pass with per-kernel events and the words copied back
+    ctx.profile(True)
+    res = ctx.compile_resident(0, copy_out=True)
+    ktimes = ctx.kernel_times()
+    ctx.profile(False)
+    stages = {k: v / 1e6 for k, v in zip(("preprocess", "isn_cnt", "isn_gen", "optimize", "regalloc",
+                                          "fix_jumps", "postprocess"), res["stage_ns"])}
+    dom = max(ktimes.items(), key=lambda kv: kv[1][0])
+    dname, (dns, dlaunch) = dom
+    alg = kernel_alg_bytes(dname, tree.n, res["W0"], res["W1"], res["n_words"], res["n_funcs"])
+    avg_ns = dns / max(1, dlaunch)
+    achieved = (alg / (avg_ns * 1e-9) / 1e9) if alg else None
+    roof = {"kernel": dname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "avg_launch_us": avg_ns / 1e3, "launches": dlaunch, "alg_bytes_per_launch": alg}
+    log("kernels (ms total, launches): " + ", ".join(
+        f"{k}={v[0] / 1e6:.3f}/{v[1]}" for k, v in sorted(ktimes.items(), key=lambda kv: -kv[1][0])))
+
+    cpu = None
+    bit_exact = None
+    if not args.no_cpu_baseline:
+        import oracle
+        c0 = time.perf_counter()
+        want = oracle.compile_tree(tree)
+        c1 = time.perf_counter()
+        cpu = {"value": tree.n / (c1 - c0), "unit": "nodes/s", "cores": 1, "kind": "port",
+               "sample": f"full {args.workload} input ({tree.n} nodes), one run of the oracle (C restatement of "
+                         f"rv_generator_avx at one thread), {c1 - c0:.1f}s"}
+        bit_exact = all(np.array_equal(res[k], want[k]) for k in
+                        ("words", "func_ids", "func_starts", "func_ends", "func_sizes"))
+        log(f"cpu oracle: {c1 - c0:.1f}s; bit-exact: {bit_exact}")
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": args.workload, "nodes": tree.n, "functions": tree.n_functions,
+                   "max_depth": tree.max_depth, "words": res["n_words"], "path": "plain one-program GPU pipeline",
+                   "inputs": "resident in HBM"},
+        "roofline": roof, "cpu_baseline": cpu, "bit_exact": bit_exact, "stages_ms": stages,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

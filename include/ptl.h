@@ -95,6 +95,18 @@ int64_t ptl_debug_dump(ptl_ctx *ctx, const char *name, uint32_t *dst, uint64_t c
  * ptl_compile (0 = all stages); for stage-boundary parity tests. */
 int ptl_debug_stop_after(ptl_ctx *ctx, int stage);
 
+/* Per-kernel timing: with profiling on, every kernel launch of the following
+ * calls is bracketed by HIP events on the context's stream.  ptl_profile_get
+ * returns, for entry idx (0-based, by first launch), the kernel name, the
+ * summed duration over all its launches in the last compile, and the number
+ * of launches; PTL_ERR_ARG past the last entry. */
+int ptl_profile_enable(ptl_ctx *ctx, int on);
+int ptl_profile_get(ptl_ctx *ctx, int idx, const char **name, uint64_t *total_ns, uint32_t *launches);
+
+/* The device exclusive-scan primitive on its own (host arrays in and out),
+ * for the known-answer test of benchmarks/prefix_sum.cpp:35-40,215-226. */
+int ptl_debug_exclusive_scan(ptl_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,8 @@ ERRORS = {
 
 # every symbol include/ptl.h declares
 EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_upload", "ptl_compile_resident",
-           "ptl_result_device", "ptl_compile_batch", "ptl_debug_dump", "ptl_debug_stop_after")
+           "ptl_result_device", "ptl_compile_batch", "ptl_debug_dump", "ptl_debug_stop_after",
+           "ptl_debug_exclusive_scan", "ptl_profile_enable", "ptl_profile_get")
 
 
 class PtlAst(C.Structure):
@@ -57,4 +58,11 @@ def bind(lib):
     lib.ptl_debug_dump.restype = C.c_int64
     lib.ptl_debug_stop_after.argtypes = [C.c_void_p, C.c_int]
     lib.ptl_debug_stop_after.restype = C.c_int
+    lib.ptl_debug_exclusive_scan.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+    lib.ptl_debug_exclusive_scan.restype = C.c_int
+    lib.ptl_profile_enable.argtypes = [C.c_void_p, C.c_int]
+    lib.ptl_profile_enable.restype = C.c_int
+    lib.ptl_profile_get.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint32)]
+    lib.ptl_profile_get.restype = C.c_int
     return lib

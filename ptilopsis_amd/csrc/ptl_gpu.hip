@@ -75,25 +75,80 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block) {
     return (uint32_t)std::min<uint64_t>(g, 0x7FFFFFFFull);
 }
 
-// --------------------------------------------------- decoupled look-back scan
-// Exclusive u32 scan over n elements.  Each 256-thread workgroup takes a 2048
-// element tile (dynamic tile ids give forward progress), scans it with wave64
-// shuffles and LDS, and chains tiles through a status word per tile
-// (bit 32: aggregate, bit 33: inclusive prefix).  Op::value(i) produces the
-// element, Op::emit(i, exclusive, value) consumes the result.
+// ------------------------------------------------------- reduce-then-scan
+// Exclusive u32 scan over n elements in three launches with no inter-block
+// waiting: per-tile sums, one workgroup scanning the tile sums, then each tile
+// rescans its elements with its offset.  A tile is 256 threads x 16 items; each
+// thread owns 16 consecutive elements, a wave64 shuffle scan and LDS combine
+// them.  Op::value(i) produces element i, Op::emit(i, exclusive, value)
+// consumes the result.
 constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_ITEMS = 16;
 constexpr uint64_t SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 
+DEV uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x;
+}
+
+DEV uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
 template <class Op>
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(Op op, uint64_t n, uint64_t *status, uint32_t *tile_ctr,
-                                                      uint32_t *total) {
-    __shared__ uint32_t s_tile, s_prefix;
-    __shared__ uint32_t s_wsum[SCAN_BLOCK / 64];
-    if (threadIdx.x == 0) s_tile = atomicAdd(tile_ctr, 1u);
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(Op op, uint64_t n, uint32_t *tile_sums) {
+    __shared__ uint32_t s_w[SCAN_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        uint64_t i = base + k;
+        if (i < n) sum += op.value(i);
+    }
+    sum = wave_sum(sum);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = sum;
     __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t base = (uint64_t)tile * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < SCAN_BLOCK / 64; ++w) t += s_w[w];
+        tile_sums[blockIdx.x] = t;
+    }
+}
+
+// One workgroup: exclusive scan of the tile sums in place; the grand total to *total.
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t *sums, uint32_t ntiles, uint32_t *total) {
+    __shared__ uint32_t s_w[16];
+    __shared__ uint32_t s_carry;
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < ntiles; b += 1024) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < ntiles ? sums[i] : 0u;
+        const uint32_t x = wave_incl_scan(v, lane);
+        if (lane == 63) s_w[wid] = x;
+        __syncthreads();
+        uint32_t pre = s_carry, all = 0;
+        for (uint32_t w = 0; w < 16; ++w) { if (w < wid) pre += s_w[w]; all += s_w[w]; }
+        if (i < ntiles) sums[i] = pre + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total) *total = s_carry;
+}
+
+template <class Op>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(Op op, uint64_t n, const uint32_t *tile_pre) {
+    __shared__ uint32_t s_w[SCAN_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t sum = 0;
 #pragma unroll
@@ -103,42 +158,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(Op op, uint64_t n, uint64_t
         sum += v[k];
     }
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-    uint32_t x = sum;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    if (lane == 63) s_wsum[wid] = x;
+    const uint32_t x = wave_incl_scan(sum, lane);
+    if (lane == 63) s_w[wid] = x;
     __syncthreads();
-    uint32_t wpre = 0, tsum = 0;
+    uint32_t run = tile_pre[blockIdx.x] + x - sum;
 #pragma unroll
-    for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
-        if ((uint32_t)w < wid) wpre += s_wsum[w];
-        tsum += s_wsum[w];
-    }
-    if (threadIdx.x == 0) {
-        uint32_t excl = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&status[0], (2ull << 32) | tsum, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&status[tile], (1ull << 32) | tsum, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t p = (int64_t)tile - 1;
-            for (;;) {
-                uint64_t s = __hip_atomic_load(&status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                uint32_t flag = (uint32_t)(s >> 32);
-                if (flag == 0) { __builtin_amdgcn_s_sleep(1); continue; }
-                excl += (uint32_t)s;
-                if (flag == 2) break;
-                --p;
-            }
-            __hip_atomic_store(&status[tile], (2ull << 32) | (excl + tsum), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_prefix = excl;
-        if ((uint64_t)(tile + 1) * SCAN_TILE >= n && total) *total = excl + tsum;
-    }
-    __syncthreads();
-    uint32_t run = s_prefix + wpre + (x - sum);
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) if ((uint32_t)w < wid) run += s_w[w];
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         uint64_t i = base + k;
@@ -694,33 +719,45 @@ struct RaStep {
     uint64_t newlife;
 };
 
-// One allocation step of one function (rv_generator_avx.cpp:1062-1397) with the
-// swapped states of its two operands given.
-DEV RaStep ra_eval(const Dev &d, uint32_t off, uint64_t life, bool s1, bool s2) {
+// One allocation step of one function (rv_generator_avx.cpp:1062-1397), given
+// the word's fields, the operands' gathered registers (raw1/raw2: 0 when not
+// virtual) and their swapped states.
+DEV RaStep ra_eval_raw(uint32_t ins, uint32_t ord, uint32_t rs1, uint32_t rs2, uint32_t raw1, uint32_t raw2,
+                       bool s1, bool s2, uint64_t life) {
     RaStep r;
-    const uint32_t ins = d.w0_instr[off];
-    r.rs1 = d.w0_rs1[off]; r.rs2 = d.w0_rs2[off]; r.ord = d.w0_rd[off];
-    const bool v1 = (int32_t)r.rs1 > 63 && r.rs1 - 64u < d.W0, v2 = (int32_t)r.rs2 > 63 && r.rs2 - 64u < d.W0;
-    if (((int32_t)r.rs1 > 63 && !v1) || ((int32_t)r.rs2 > 63 && !v2)) atomicOr(d.flags, EF_OOB);
-    r.r1r = v1 ? d.sym_reg[r.rs1 - 64u] : 0u;
-    r.r2r = v2 ? d.sym_reg[r.rs2 - 64u] : 0u;
+    r.rs1 = rs1; r.rs2 = rs2; r.ord = ord;
+    const bool v1 = (int32_t)rs1 > 63, v2 = (int32_t)rs2 > 63;
+    r.r1r = v1 ? raw1 : 0u;
+    r.r2r = v2 ? raw2 : 0u;
     if (v1 && s1) r.r1r = (ins == 0xC0007053u || (ins & 0x7Fu) == 0x53u) ? 37u : 5u;
     if (v2 && s2) r.r2r = ((ins & 0x7Fu) == 0x53u) ? 38u : 6u;
     const uint64_t clr = ((1ull << r.r1r) | (1ull << r.r2r)) & ~1ull;
     const uint64_t cl = life & ~clr;
-    uint32_t rdr = r.ord;
-    if ((int32_t)r.ord > 63) {
+    uint32_t rdr = ord;
+    if ((int32_t)ord > 63) {
         const bool rdf = ins == 0xD0007053u || (ins & 0x7Fu) == 0x53u;
         rdr = ffz_gcc(cl | (rdf ? 0xFFFFFFFFull : 0xFFFFFFFF00000000ull));
         if (rdr == 64) rdr = rdf ? 37u : 5u;
     }
     r.rdr = rdr;
-    const uint64_t rdm = 1ull << rdr;
     r.sw1 = r.r1r != 0 && ((life >> r.r1r) & 1ull);
     r.sw2 = r.r2r != 0 && ((life >> r.r2r) & 1ull);
     r.swd = rdr != 0 && ((cl >> rdr) & 1ull);
-    r.newlife = cl | rdm;
+    r.newlife = cl | (1ull << rdr);
     return r;
+}
+
+DEV bool virt_ok(const Dev &d, uint32_t v) {
+    if ((int32_t)v <= 63) return false;
+    if (v - 64u >= d.W0) { atomicOr(d.flags, EF_OOB); return false; }
+    return true;
+}
+
+DEV RaStep ra_eval(const Dev &d, uint32_t off, uint64_t life, bool s1, bool s2) {
+    const uint32_t ins = d.w0_instr[off], rs1 = d.w0_rs1[off], rs2 = d.w0_rs2[off], ord = d.w0_rd[off];
+    const uint32_t raw1 = virt_ok(d, rs1) ? d.sym_reg[rs1 - 64u] : 0u;
+    const uint32_t raw2 = virt_ok(d, rs2) ? d.sym_reg[rs2 - 64u] : 0u;
+    return ra_eval_raw(ins, ord, rs1, rs2, raw1, raw2, s1, s2, life);
 }
 
 DEV void ra_commit_syms(const Dev &d, const RaStep &r, bool reset_sw) {
@@ -795,7 +832,7 @@ DEV RaStep ra0_eval(const Dev &d, uint32_t off, uint64_t life, bool f1, bool f2)
     return ra_eval(d, off, life, s1, s2);
 }
 
-__global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0(Dev d, uint32_t maxsize, const uint32_t *gmin,
+__global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_t maxsize, const uint32_t *gmin,
                                                          const uint32_t *gmax, uint64_t *life0, Ra0Rec *rec) {
     __shared__ int32_t Rv[64], Ro[64], Sv[64], So[64];
     __shared__ uint32_t Lmin[64], Lmax[64], fbits[64], wkey[64];
@@ -884,6 +921,208 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0(Dev d, uint32_t maxsiz
                     if (reset) mark = Lmax[tid] > o + 1u;
                 }
             }
+            if (mark) {
+                if (v - 64u < d.W0) d.sym_sw[v - 64u] = 1;
+                else atomicOr(d.flags, EF_OOB);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Register-resident lockstep: each thread owns PER lane-0 functions (j = tid +
+// q*blockDim) and keeps their state, the current step's fields and the result
+// of evaluating it in registers; the next step's fields are prefetched.
+// Settling an intra-step dependency re-evaluates only functions whose operand
+// flips changed, from cached inputs.  Owners of snapshot entries are found by a
+// 128-slot open-addressing table in LDS.
+struct Ra0Fn {
+    uint32_t start, size;
+    uint64_t life;
+    uint32_t ins, ord, rs1, rs2;
+    uint32_t n_ins, n_ord, n_rs1, n_rs2;
+    uint32_t pk;  // raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18
+};
+
+DEV RaStep ra0_step(const Ra0Fn &f) {
+    const uint32_t fl = f.pk >> 18;
+    return ra_eval_raw(f.ins, f.ord, f.rs1, f.rs2, f.pk & 0xFFu, (f.pk >> 8) & 0xFFu,
+                       ((f.pk >> 16) & 1u) || (fl & 1u), ((f.pk >> 17) & 1u) || (fl & 2u), f.life);
+}
+
+DEV void ot_insert(int32_t *key, uint32_t *bits, int32_t o, uint32_t b) {
+    uint32_t h = (uint32_t)o & 127u;
+    for (int k = 0; k < 128; ++k) {
+        int32_t prev = atomicCAS(&key[h], -1, o);
+        if (prev == -1 || prev == o) { atomicOr(&bits[h], b); return; }
+        h = (h + 1) & 127u;
+    }
+}
+
+DEV uint32_t ot_lookup(const int32_t *key, const uint32_t *bits, int32_t o) {
+    uint32_t h = (uint32_t)o & 127u;
+    for (int k = 0; k < 128; ++k) {
+        int32_t kk = key[h];
+        if (kk == o) return bits[h];
+        if (kk == -1) return 0;
+        h = (h + 1) & 127u;
+    }
+    return 0;
+}
+
+constexpr uint32_t RA0_LDS_FNS = 8192;  // per-function fields kept in LDS up to this many lane-0 functions
+
+template <int PER>
+__global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, const uint32_t *gmin,
+                                                   const uint32_t *gmax, uint32_t *gcur /* 3*L */) {
+    __shared__ uint32_t s_cur[3 * RA0_LDS_FNS];
+    __shared__ int32_t Rv[64], Ro[64], Sv[64], So[64];
+    __shared__ uint32_t Lmin[64], Lmax[64], Bmin[64], Bmax[64], fbits[64], wkey[64];
+    __shared__ int32_t ot_key[128];
+    __shared__ uint32_t ot_bits[128];
+    __shared__ int changed;
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    const uint32_t L = (d.F + 7u) / 8u;
+    uint32_t *cur = L <= RA0_LDS_FNS ? s_cur : gcur;
+    uint32_t *cur_rd = cur, *cur_rs1 = cur + L, *cur_rs2 = cur + 2 * L;
+    Ra0Fn fn[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const uint32_t j = tid + q * nth;
+        fn[q].start = 0; fn[q].size = 0; fn[q].life = 0x1Full; fn[q].pk = 0;
+        fn[q].n_ins = fn[q].n_ord = fn[q].n_rs1 = fn[q].n_rs2 = 0;
+        if (j < L) {
+            fn[q].start = d.fstarts[8u * j];
+            fn[q].size = d.fsizes[8u * j];
+            if ((int32_t)fn[q].size > 0) {
+                const uint32_t off = fn[q].start;
+                fn[q].n_ins = d.w0_instr[off]; fn[q].n_ord = d.w0_rd[off];
+                fn[q].n_rs1 = d.w0_rs1[off]; fn[q].n_rs2 = d.w0_rs2[off];
+            }
+        }
+    }
+    uint32_t nmin = 0xFFFFFFFFu, nmax = 0;
+    if (tid < 64) { Rv[tid] = -1; Ro[tid] = -1; if (maxsize) { nmin = gmin[tid]; nmax = gmax[tid]; } }
+    __syncthreads();
+    for (uint32_t i = 0; i < maxsize; ++i) {
+        if (tid < 64) {
+            Sv[tid] = Rv[tid]; So[tid] = Ro[tid]; fbits[tid] = 0; wkey[tid] = 0;
+            Bmin[tid] = nmin; Bmax[tid] = nmax; Lmin[tid] = nmin; Lmax[tid] = nmax;
+            if (i + 1 < maxsize) { nmin = gmin[(uint64_t)(i + 1) * 64u + tid]; nmax = gmax[(uint64_t)(i + 1) * 64u + tid]; }
+        }
+        if (tid == 0) changed = 0;
+        // this step's fields (prefetched), the operands' state, next step's prefetch
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t j = tid + q * nth;
+            const bool act = j < L && (int32_t)i < (int32_t)fn[q].size;
+            fn[q].ins = fn[q].n_ins; fn[q].ord = fn[q].n_ord; fn[q].rs1 = fn[q].n_rs1; fn[q].rs2 = fn[q].n_rs2;
+            if (j < L) {
+                cur_rd[j] = act ? fn[q].ord : 0u; cur_rs1[j] = act ? fn[q].rs1 : 0u; cur_rs2[j] = act ? fn[q].rs2 : 0u;
+            }
+            if (act) {
+                const bool v1 = virt_ok(d, fn[q].rs1), v2 = virt_ok(d, fn[q].rs2);
+                const uint32_t raw1 = v1 ? d.sym_reg[fn[q].rs1 - 64u] : 0u;
+                const uint32_t raw2 = v2 ? d.sym_reg[fn[q].rs2 - 64u] : 0u;
+                const uint32_t s1 = (v1 && d.sym_sw[fn[q].rs1 - 64u]) ? 1u : 0u;
+                const uint32_t s2 = (v2 && d.sym_sw[fn[q].rs2 - 64u]) ? 1u : 0u;
+                fn[q].pk = raw1 | (raw2 << 8) | (s1 << 16) | (s2 << 17);
+                if ((int32_t)(i + 1) < (int32_t)fn[q].size) {
+                    const uint32_t off = fn[q].start + i + 1;
+                    fn[q].n_ins = d.w0_instr[off]; fn[q].n_ord = d.w0_rd[off];
+                    fn[q].n_rs1 = d.w0_rs1[off]; fn[q].n_rs2 = d.w0_rs2[off];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t j = tid + q * nth;
+            if (j < L && (int32_t)i < (int32_t)fn[q].size) {
+                const RaStep r = ra0_step(fn[q]);
+                if (r.swd) { atomicMin(&Lmin[r.rdr], j); atomicMax(&Lmax[r.rdr], j + 1u); }
+                if (r.sw1) { atomicMin(&Lmin[r.r1r], j); atomicMax(&Lmax[r.r1r], j + 1u); }
+                if (r.sw2) { atomicMin(&Lmin[r.r2r], j); atomicMax(&Lmax[r.r2r], j + 1u); }
+            }
+        }
+        __syncthreads();
+        for (uint32_t round = 0;; ++round) {
+            if (tid < 128) { ot_key[tid] = -1; ot_bits[tid] = 0; }
+            __syncthreads();
+            if (tid < 64) {
+                uint32_t nb = 0;
+                const int32_t o = So[tid];
+                if (Sv[tid] >= 64 && o >= 0 && Lmin[tid] < (uint32_t)o) {
+                    if (cur_rs1[o] == (uint32_t)Sv[tid]) nb |= 1u;
+                    if (cur_rs2[o] == (uint32_t)Sv[tid]) nb |= 2u;
+                }
+                if (nb) ot_insert(ot_key, ot_bits, o, nb);
+                if (nb != fbits[tid]) { fbits[tid] = nb; changed = 1; }
+            }
+            __syncthreads();
+            int again = changed;
+            if (again && round > L + 1) { again = 0; if (tid == 0) atomicOr(d.flags, EF_OOB); }
+            if (!again) break;
+            __syncthreads();
+            if (tid < 64) { Lmin[tid] = Bmin[tid]; Lmax[tid] = Bmax[tid]; }
+            if (tid == 0) changed = 0;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const uint32_t j = tid + q * nth;
+                if (j < L && (int32_t)i < (int32_t)fn[q].size) {
+                    const uint32_t f = ot_lookup(ot_key, ot_bits, (int32_t)j);
+                    fn[q].pk = (fn[q].pk & 0x3FFFFu) | (f << 18);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const uint32_t j = tid + q * nth;
+                if (j < L && (int32_t)i < (int32_t)fn[q].size) {
+                    const RaStep r = ra0_step(fn[q]);
+                    if (r.swd) { atomicMin(&Lmin[r.rdr], j); atomicMax(&Lmax[r.rdr], j + 1u); }
+                    if (r.sw1) { atomicMin(&Lmin[r.r1r], j); atomicMax(&Lmax[r.r1r], j + 1u); }
+                    if (r.sw2) { atomicMin(&Lmin[r.r2r], j); atomicMax(&Lmax[r.r2r], j + 1u); }
+                }
+            }
+            __syncthreads();
+        }
+        // commit: lifetime, region-0 write keys (group order: the last writer wins)
+        uint32_t rr[PER];  // committed r1r | r2r << 8 | rdr << 16
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t j = tid + q * nth;
+            rr[q] = 0;
+            if (j < L && (int32_t)i < (int32_t)fn[q].size) {
+                const RaStep r = ra0_step(fn[q]);
+                rr[q] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
+                fn[q].life = r.newlife;
+                if (r.ord) atomicMax(&wkey[r.rdr], ((j + 1u) << 2) | 1u);
+                if (r.r1r != r.rdr) atomicMax(&wkey[r.r1r], ((j + 1u) << 2) | 2u);
+                if (r.r2r != r.rdr) atomicMax(&wkey[r.r2r], ((j + 1u) << 2) | 3u);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t j = tid + q * nth;
+            if (j < L && (int32_t)i < (int32_t)fn[q].size) {
+                RaStep r;
+                r.ord = fn[q].ord; r.rs1 = fn[q].rs1; r.rs2 = fn[q].rs2;
+                r.r1r = rr[q] & 0xFFu; r.r2r = (rr[q] >> 8) & 0xFFu; r.rdr = (rr[q] >> 16) & 0xFFu;
+                if (r.ord && wkey[r.rdr] == (((j + 1u) << 2) | 1u)) { Rv[r.rdr] = (int32_t)r.ord; Ro[r.rdr] = (int32_t)j; }
+                if (r.r1r != r.rdr && wkey[r.r1r] == (((j + 1u) << 2) | 2u)) { Rv[r.r1r] = -1; Ro[r.r1r] = -1; }
+                if (r.r2r != r.rdr && wkey[r.r2r] == (((j + 1u) << 2) | 3u)) { Rv[r.r2r] = -1; Ro[r.r2r] = -1; }
+                ra_commit_syms(d, r, true);
+            }
+        }
+        __syncthreads();
+        // marks on the snapshot's virtual registers
+        if (tid < 64 && Sv[tid] >= 64 && Lmax[tid] != 0) {
+            const uint32_t v = (uint32_t)Sv[tid];
+            bool mark = true;
+            const int32_t o = So[tid];
+            if (o >= 0 && (cur_rd[o] == v || cur_rs1[o] == v || cur_rs2[o] == v)) mark = Lmax[tid] > (uint32_t)o + 1u;
             if (mark) {
                 if (v - 64u < d.W0) d.sym_sw[v - 64u] = 1;
                 else atomicOr(d.flags, EF_OOB);

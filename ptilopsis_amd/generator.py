@@ -74,8 +74,28 @@ class Context:
             raise PtlError(s)
         return result_dict(r, copy_out)
 
+    def profile(self, on: bool = True) -> None:
+        self.lib.ptl_profile_enable(self.h, 1 if on else 0)
+
+    def kernel_times(self) -> dict:
+        """{kernel name: (total ns, launches)} of the last compile with profiling on."""
+        out, i = {}, 0
+        name, ns, cnt = C.c_char_p(), C.c_uint64(), C.c_uint32()
+        while self.lib.ptl_profile_get(self.h, i, C.byref(name), C.byref(ns), C.byref(cnt)) == 0:
+            out[name.value.decode()] = (int(ns.value), int(cnt.value))
+            i += 1
+        return out
+
     def stop_after(self, stage: int) -> None:
         self.lib.ptl_debug_stop_after(self.h, stage)
+
+    def exclusive_scan(self, x: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(x, dtype=np.uint32)
+        out = np.zeros_like(a)
+        s = self.lib.ptl_debug_exclusive_scan(self.h, a.ctypes.data, out.ctypes.data, a.shape[0])
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return out
 
     def dump(self, name: str) -> np.ndarray:
         n = self.lib.ptl_debug_dump(self.h, name.encode(), None, 0)

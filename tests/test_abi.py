@@ -1,0 +1,39 @@
+"""The C ABI library loads and exports every entry point include/ptl.h declares."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HIP_SO = os.path.join(ROOT, "ptilopsis_amd", "libptl_hip.so")
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "ptl.h")).read()
+    return sorted(set(re.findall(r"\b(ptl_[a-z_]+)\s*\(", txt)))
+
+
+def test_header_declares_the_abi():
+    names = declared()
+    for must in ("ptl_create", "ptl_compile", "ptl_compile_batch", "ptl_destroy", "ptl_strerror"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(HIP_SO):
+        pytest.skip("libptl_hip.so not built (run __graft_entry__.build())")
+    lib = C.CDLL(HIP_SO)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing
+    from ptilopsis_amd import abi
+    assert set(abi.EXPORTS) == set(declared())
+
+
+def test_strerror_without_a_device():
+    if not os.path.exists(HIP_SO):
+        pytest.skip("libptl_hip.so not built")
+    lib = C.CDLL(HIP_SO)
+    lib.ptl_strerror.restype = C.c_char_p
+    assert lib.ptl_strerror(0) == b"ok"
+    assert b"DepthTree" in lib.ptl_strerror(-2)

@@ -1,0 +1,157 @@
+"""GPU parity: libptl_hip.so against the oracle, bit-exact, through the C ABI.
+
+Small and medium programs are compared word for word (and stage by stage for
+the corpus); large ones through size-independent properties.  Each case prints
+what it runs before touching the GPU."""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from ptilopsis_amd import parse_source, workloads as W
+from ptilopsis_amd.generator import Context, PtlError
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+CORPUS = sorted(glob.glob(os.path.join(HERE, "golden", "corpus", "*")))
+OUT = ("words", "func_ids", "func_starts", "func_ends", "func_sizes")
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def assert_same(gpu, ref):
+    for k in OUT:
+        g, r = np.asarray(gpu[k]), np.asarray(ref[k])
+        assert g.shape == r.shape, (k, g.shape, r.shape)
+        bad = np.nonzero(g != r)[0]
+        assert bad.size == 0, f"{k}: {bad.size} differ, first at {bad[0]}: {g[bad[0]]:#x} != {r[bad[0]]:#x}"
+
+
+def test_gpu_scan_kat(gpu_ctx):
+    import mt19937_64 as MT
+    x = MT.uniform_0_4095(0xC0FFEEA4DBEEF, 1 << 20)
+    say("scan KAT 2^20")
+    ex = gpu_ctx.exclusive_scan(x)
+    ref = np.cumsum(x.astype(np.uint64))
+    assert ex[0] == 0 and np.array_equal(ex[1:].astype(np.uint64), ref[:-1])
+
+
+@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, 1 << 22])
+def test_gpu_scan_sizes(gpu_ctx, n):
+    x = np.random.default_rng(n).integers(0, 1000, n).astype(np.uint32)
+    ex = gpu_ctx.exclusive_scan(x)
+    assert np.array_equal(ex, (np.cumsum(x.astype(np.uint64)) - x).astype(np.uint32))
+
+
+STAGE_ARRAYS = {1: ("types", "rtypes", "node_data"), 2: ("locs", "func_ids", "func_starts", "func_ends", "func_sizes"),
+                3: ("instr", "rd", "rs1", "rs2", "jt"), 4: ("used",), 5: ("sym_sw", "instr", "rd", "rs1", "rs2", "jt")}
+
+
+def norm(name, a):
+    a = np.asarray(a, np.uint64)
+    if name in ("used", "sym_sw"):
+        return (a != 0).astype(np.uint64)
+    if name in ("types", "rtypes"):
+        return a & 0xFF
+    return a
+
+
+@pytest.mark.parametrize("path", CORPUS, ids=[os.path.basename(p) for p in CORPUS])
+def test_gpu_corpus_stage_by_stage(gpu_ctx, path):
+    t = parse_source(open(path, "rb").read())
+    say("corpus", os.path.basename(path), t.n)
+    o = oracle.Oracle(t)
+    try:
+        for st in range(1, 6):
+            o.run(st)
+            gpu_ctx.stop_after(st)
+            gpu_ctx.compile(t)
+            for nm in STAGE_ARRAYS[st]:
+                assert np.array_equal(norm(nm, gpu_ctx.dump(nm)), norm(nm, o.get(nm))), (st, nm)
+    finally:
+        gpu_ctx.stop_after(0)
+    o.run(7)
+    assert_same(gpu_ctx.compile(t), o.result())
+
+
+GEN_CASES = [("base", s, 4 + s) for s in range(6)] + [("stress", 100 + s, 2) for s in range(3)] + \
+            [("manyfloat", 200 + s, 8) for s in range(2)]
+
+
+@pytest.mark.parametrize("kind,seed,nf", GEN_CASES, ids=[f"{k}-{s}" for k, s, _ in GEN_CASES])
+def test_gpu_generated(gpu_ctx, kind, seed, nf):
+    base = {"base": W.BASE, "stress": W.STRESS, "manyfloat": dict(W.BASE, float_frac=0.9)}[kind]
+    src, n = W.program_source(seed, nf, base=base)
+    t = parse_source(src)
+    say("generated", kind, seed, t.n)
+    assert_same(gpu_ctx.compile(t), oracle.compile_tree(t))
+
+
+def test_gpu_single_50k(gpu_ctx):
+    t = W.single_50k()
+    say("single_50k", t.n)
+    r = gpu_ctx.compile(t)
+    assert r["tie_slots"] > 0  # exercises the introsort ordering of equal-depth writers
+    assert_same(r, oracle.compile_tree(t))
+
+
+def test_gpu_one_million_nodes(gpu_ctx):
+    t = W.program_of_size(0x5EED0007, 1_000_000)
+    say("1M-node program", t.n)
+    assert_same(gpu_ctx.compile(t), oracle.compile_tree(t))
+
+
+def test_gpu_many_small_functions(gpu_ctx):
+    src, _ = W.program_source(31, 300, min_stmts=1, max_stmts=3)
+    t = parse_source(src)
+    say("300 tiny functions", t.n)
+    assert_same(gpu_ctx.compile(t), oracle.compile_tree(t))
+
+
+def test_gpu_single_function_and_single_statement(gpu_ctx):
+    for src in ("function f() : void { }", "function f(a : int) : int { return a; }",
+                "function f(a : float, b : int) : float { while a < 2.0 { a = a + 1.0; } return a; }"):
+        t = parse_source(src)
+        assert_same(gpu_ctx.compile(t), oracle.compile_tree(t))
+
+
+def test_gpu_errors(gpu_ctx):
+    empty = parse_source("")
+    with pytest.raises(PtlError) as e:
+        gpu_ctx.compile(empty)
+    assert e.value.status == -3
+    t = W.single_50k()
+    bad = W.Tree(t.node_types, t.result_types, t.parents.copy(), t.depth, t.child_idx, t.node_data, t.max_depth)
+    bad.parents[10] = 5  # a child before its parent breaks post-order
+    with pytest.raises(PtlError) as e:
+        gpu_ctx.compile(bad)
+    assert e.value.status == -2
+
+
+def test_gpu_resident_deterministic(gpu_ctx):
+    t = W.program_of_size(0x5EED0008, 300_000)
+    a = gpu_ctx.compile(t)
+    gpu_ctx.upload(3, t)
+    b = gpu_ctx.compile_resident(3)
+    c = gpu_ctx.compile_resident(3)
+    assert_same(a, b)
+    assert_same(b, c)
+
+
+def test_gpu_large_properties(gpu_ctx):
+    # size-independent properties at 4M nodes: the function table tiles the
+    # output, functions open with the prologue stores and close with ret
+    t = W.sized_program(0x5EED0009, 4_000_000, 5000)
+    say("4M-node program", t.n)
+    r = gpu_ctx.compile(t)
+    w, st, en, sz = r["words"], r["func_starts"], r["func_ends"], r["func_sizes"]
+    assert st[0] == 0 and en[-1] == len(w) and np.array_equal(st[1:], en[:-1]) and np.array_equal(sz, en - st)
+    assert np.all(w[st] == 0xFE112E23) and np.all(w[st + 1] == 0xFE812C23) and np.all(w[en - 1] == 0x8067)
+    assert np.array_equal(np.sort(r["func_ids"]), np.arange(len(st)))
+    r2 = gpu_ctx.compile(t)
+    assert_same(r, r2)
