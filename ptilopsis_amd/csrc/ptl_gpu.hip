@@ -51,9 +51,21 @@ DEV uint32_t spec_i(uint32_t ty, uint32_t o, uint32_t rt) { return ((uint32_t)c_
 DEV uint32_t size_of(uint32_t ty, uint32_t rt) { return (c_bits[spec_i(ty, 0, rt)] >> 19) & 7u; }
 
 // error flags raised by kernels (bit set in ctx->d_flags)
-enum : uint32_t { EF_OOB = 1, EF_XFUNC = 2, EF_CALLPATH = 4 };
+// EF_XNAMED: some function names a virtual register of a word outside itself;
+// EF_XCONFLICT: such a register is also named by another function (see xref_note).
+enum : uint32_t { EF_OOB = 1, EF_CALLPATH = 4, EF_XNAMED = 16, EF_XCONFLICT = 32 };
 
 // ------------------------------------------------------------------ helpers
+// Workgroup barrier that orders LDS only: global loads and stores stay in
+// flight across it.  Correct where every global location a thread reads was
+// last written by that same thread (program order covers it) or before the
+// kernel started, and where global atomics are read only by later kernels.
+DEV void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 DEV uint32_t ffz_gcc(uint64_t v) {
     // include/utils.hpp:25-45 as compiled by GCC: __builtin_ffs(int) of ~val sees 32 bits.
     uint32_t low = (uint32_t)(~v);
@@ -196,7 +208,9 @@ struct Dev {
     // optimize
     uint8_t *used, *sstore, *usedi;
     // regalloc
-    uint8_t *sym_reg, *sym_sw;
+    uint4 *w0_f4;            // per word: rd, rs1, rs2, instruction (one 16-byte load)
+    uint16_t *sym;           // per virtual register: assigned register | swapped << 8
+    uint32_t *xref;          // per virtual register named from outside its word's function: namer + 1
     uint32_t *ra_off;
     uint32_t W1;
     uint32_t *w1_instr, *w1_rd, *w1_rs1, *w1_rs2, *w1_jt;
@@ -709,7 +723,9 @@ __global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
 __global__ __launch_bounds__(256) void k_opt_final(Dev d) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= d.W0) return;
-    d.usedi[i] = (!((int32_t)d.w0_rd[i] > 63) || d.used[i]) ? 1 : 0;
+    const uint32_t rd = d.w0_rd[i];
+    d.usedi[i] = (!((int32_t)rd > 63) || d.used[i]) ? 1 : 0;
+    d.w0_f4[i] = make_uint4(rd, d.w0_rs1[i], d.w0_rs2[i], d.w0_instr[i]);  // regalloc's view of the word
 }
 
 // ================================================================= regalloc
@@ -754,28 +770,62 @@ DEV bool virt_ok(const Dev &d, uint32_t v) {
 }
 
 DEV RaStep ra_eval(const Dev &d, uint32_t off, uint64_t life, bool s1, bool s2) {
-    const uint32_t ins = d.w0_instr[off], rs1 = d.w0_rs1[off], rs2 = d.w0_rs2[off], ord = d.w0_rd[off];
-    const uint32_t raw1 = virt_ok(d, rs1) ? d.sym_reg[rs1 - 64u] : 0u;
-    const uint32_t raw2 = virt_ok(d, rs2) ? d.sym_reg[rs2 - 64u] : 0u;
-    return ra_eval_raw(ins, ord, rs1, rs2, raw1, raw2, s1, s2, life);
+    const uint4 w = d.w0_f4[off];
+    const uint32_t raw1 = virt_ok(d, w.y) ? (d.sym[w.y - 64u] & 0xFFu) : 0u;
+    const uint32_t raw2 = virt_ok(d, w.z) ? (d.sym[w.z - 64u] & 0xFFu) : 0u;
+    return ra_eval_raw(w.w, w.x, w.y, w.z, raw1, raw2, s1, s2, life);
 }
 
-DEV void ra_commit_syms(const Dev &d, const RaStep &r, bool reset_sw) {
+DEV bool sym_swapped(const Dev &d, uint32_t v) { return (d.sym[v] >> 8) != 0; }
+
+// Register allocation keeps each virtual register's state with the one thread
+// that walks the function naming it, which is exact when no register is named
+// by two functions.  isn_gen ORs a word's own virtual register into physical
+// argument registers (rv_generator_avx.cpp:676-697 for rd; operands, which
+// read such a result from a parent slot, :842), and for calls with many
+// arguments the result can name a word of another function.  Such
+// names are noted here; the host then checks that no other function names the
+// same register (k_xref_check) and rejects the program otherwise.
+DEV void xref_note(const Dev &d, uint32_t v, uint32_t f) {
+    const uint32_t e = v - 64u;
+    if (e >= d.W0) return;  // out of range: flagged by the caller's bounds checks
+    const uint32_t prev = atomicCAS(&d.xref[e], 0u, f + 1u);
+    if (prev != 0u && prev != f + 1u) atomicOr(d.flags, EF_XCONFLICT);
+    atomicOr(d.flags, EF_XNAMED);
+}
+
+DEV void xref_check_fields(const Dev &d, uint32_t f, uint32_t start, uint32_t size, uint32_t ord, uint32_t rs1,
+                           uint32_t rs2) {
+    if ((int32_t)ord > 63 && ord - 64u - start >= size) xref_note(d, ord, f);
+    if ((int32_t)rs1 > 63 && rs1 - 64u - start >= size) xref_note(d, rs1, f);
+    if ((int32_t)rs2 > 63 && rs2 - 64u - start >= size) xref_note(d, rs2, f);
+}
+DEV void sym_mark(const Dev &d, uint32_t v) { reinterpret_cast<uint8_t *>(d.sym)[2u * v + 1u] = 1; }
+
+// register_state updates of one step (rv_generator_avx.cpp:1356-1397): each
+// named virtual register takes its register and, as the lane-0 lockstep
+// commits, a cleared swapped flag.  The swapped flags of functions outside
+// lane 0 are never set, so writing the whole entry is the same as writing the
+// register alone there.
+DEV void ra_commit_syms(const Dev &d, const RaStep &r) {
     if (((int32_t)r.ord > 63 && r.ord - 64u >= d.W0) || ((int32_t)r.rs1 > 63 && r.rs1 - 64u >= d.W0) ||
         ((int32_t)r.rs2 > 63 && r.rs2 - 64u >= d.W0)) { atomicOr(d.flags, EF_OOB); return; }
-    if ((int32_t)r.ord > 63) { d.sym_reg[r.ord - 64u] = (uint8_t)r.rdr; if (reset_sw) d.sym_sw[r.ord - 64u] = 0; }
-    if ((int32_t)r.rs1 > 63) { d.sym_reg[r.rs1 - 64u] = (uint8_t)r.r1r; if (reset_sw) d.sym_sw[r.rs1 - 64u] = 0; }
-    if ((int32_t)r.rs2 > 63) { d.sym_reg[r.rs2 - 64u] = (uint8_t)r.r2r; if (reset_sw) d.sym_sw[r.rs2 - 64u] = 0; }
+    if ((int32_t)r.ord > 63) d.sym[r.ord - 64u] = (uint16_t)r.rdr;
+    if ((int32_t)r.rs1 > 63) d.sym[r.rs1 - 64u] = (uint16_t)r.r1r;
+    if ((int32_t)r.rs2 > 63) d.sym[r.rs2 - 64u] = (uint16_t)r.r2r;
 }
 
 // Functions outside lane 0 of their group: their swapped flags are never set
 // (only lane-0 registers are ever looked up), so each walks alone.  Per step,
 // the registers each group looks up are recorded as min/max group per register.
-__global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin, uint32_t *gmax) {
+// permw lists these functions by size, largest first, so that the 64 walks of
+// a workgroup take about the same number of steps.
+__global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin, uint32_t *gmax, const uint32_t *permw,
+                                                uint32_t nw) {
     __shared__ uint32_t smin[64], smax[64];
     const uint32_t lane = threadIdx.x;
     const uint32_t t = blockIdx.x * 64u + lane;
-    const uint32_t f = 8u * (t / 7u) + 1u + (t % 7u);
+    const uint32_t f = t < nw ? permw[t] : d.F;
     const bool act = f < d.F;
     const uint32_t start = act ? d.fstarts[f] : 0u;
     const uint32_t size = act ? d.fsizes[f] : 0u;
@@ -789,20 +839,21 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin, uint32_t 
         if ((int32_t)i < (int32_t)size) {
             const uint32_t off = start + i;
             RaStep r = ra_eval(d, off, life, false, false);
+            xref_check_fields(d, f, start, size, r.ord, r.rs1, r.rs2);
             if (r.swd) { atomicMin(&smin[r.rdr], grp); atomicMax(&smax[r.rdr], grp + 1u); }
             if (r.sw1) { atomicMin(&smin[r.r1r], grp); atomicMax(&smax[r.r1r], grp + 1u); }
             if (r.sw2) { atomicMin(&smin[r.r2r], grp); atomicMax(&smax[r.r2r], grp + 1u); }
             life = r.newlife;
-            ra_commit_syms(d, r, false);
+            ra_commit_syms(d, r);
         }
-        __syncthreads();
+        lds_barrier();
         uint32_t mx = smax[lane];
         if (mx) {
             atomicMin(&gmin[(uint64_t)i * 64u + lane], smin[lane]);
             atomicMax(&gmax[(uint64_t)i * 64u + lane], mx);
             smin[lane] = 0xFFFFFFFFu; smax[lane] = 0;
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -826,9 +877,9 @@ DEV void ra0_flips(const int32_t *So, const uint32_t *fbits, uint32_t j, bool &f
 }
 
 DEV RaStep ra0_eval(const Dev &d, uint32_t off, uint64_t life, bool f1, bool f2) {
-    const uint32_t rs1 = d.w0_rs1[off], rs2 = d.w0_rs2[off];
-    bool s1 = ((int32_t)rs1 > 63 && rs1 - 64u < d.W0 && d.sym_sw[rs1 - 64u]) || f1;
-    bool s2 = ((int32_t)rs2 > 63 && rs2 - 64u < d.W0 && d.sym_sw[rs2 - 64u]) || f2;
+    const uint32_t rs1 = d.w0_f4[off].y, rs2 = d.w0_f4[off].z;
+    bool s1 = ((int32_t)rs1 > 63 && rs1 - 64u < d.W0 && sym_swapped(d, rs1 - 64u)) || f1;
+    bool s2 = ((int32_t)rs2 > 63 && rs2 - 64u < d.W0 && sym_swapped(d, rs2 - 64u)) || f2;
     return ra_eval(d, off, life, s1, s2);
 }
 
@@ -867,8 +918,8 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
                     const uint32_t f = 8u * o;
                     if ((int32_t)i < (int32_t)d.fsizes[f] && Lmin[tid] < o) {
                         const uint32_t off = d.fstarts[f] + i;
-                        if (d.w0_rs1[off] == (uint32_t)Sv[tid]) nb |= 1u;
-                        if (d.w0_rs2[off] == (uint32_t)Sv[tid]) nb |= 2u;
+                        if (d.w0_f4[off].y == (uint32_t)Sv[tid]) nb |= 1u;
+                        if (d.w0_f4[off].z == (uint32_t)Sv[tid]) nb |= 2u;
                     }
                 }
                 if (nb != fbits[tid]) changed = 1;
@@ -905,7 +956,7 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
             if (r.ord && wkey[r.rdr] == (((j + 1u) << 2) | 1u)) { Rv[r.rdr] = (int32_t)r.ord; Ro[r.rdr] = (int32_t)j; }
             if (r.r1r != r.rdr && wkey[r.r1r] == (((j + 1u) << 2) | 2u)) { Rv[r.r1r] = -1; Ro[r.r1r] = -1; }
             if (r.r2r != r.rdr && wkey[r.r2r] == (((j + 1u) << 2) | 3u)) { Rv[r.r2r] = -1; Ro[r.r2r] = -1; }
-            ra_commit_syms(d, r, true);
+            ra_commit_syms(d, r);
         }
         __syncthreads();
         // marks on the snapshot's virtual registers
@@ -922,7 +973,7 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
                 }
             }
             if (mark) {
-                if (v - 64u < d.W0) d.sym_sw[v - 64u] = 1;
+                if (v - 64u < d.W0) sym_mark(d, v - 64u);
                 else atomicOr(d.flags, EF_OOB);
             }
         }
@@ -930,24 +981,90 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
     }
 }
 
-// Register-resident lockstep: each thread owns PER lane-0 functions (j = tid +
-// q*blockDim) and keeps their state, the current step's fields and the result
-// of evaluating it in registers; the next step's fields are prefetched.
-// Settling an intra-step dependency re-evaluates only functions whose operand
-// flips changed, from cached inputs.  Owners of snapshot entries are found by a
-// 128-slot open-addressing table in LDS.
+
+// Register-resident lockstep: each thread owns PER lane-0 functions (slots
+// t = tid + q*blockDim; slot t runs group perm[t], the groups ordered by
+// function size, largest first, so that the functions still running at a step
+// fill the first waves and finished waves skip each phase) and keeps their
+// state in registers: the fields of steps i+1 and
+// i+2 (the latter in flight), the operand state of step i (packed) and of step
+// i+1 (in flight, patched with step i's commit and marks when it lands).  The
+// fields of step i sit in LDS (cur_*), where the snapshot owners' checks read
+// them too.
+// A function's virtual registers are read and written only by its own thread
+// (marks by other groups are posted to the owner through LDS), so the three
+// barriers per step order LDS alone and never wait for HBM.  Per-step LDS
+// tables are double-buffered by step parity so that no barrier is needed
+// between the end of a step and the start of the next.  Of an instruction
+// word, register allocation looks only at whether its major opcode is OP-FP
+// (0x53): the other word tests of ra_eval_raw (0xC0007053, 0xD0007053) are
+// OP-FP words themselves, so the lockstep keeps that bit alone.
 struct Ra0Fn {
-    uint32_t start, size;
+    uint32_t j, start, size;       // group, first word, word count
     uint64_t life;
-    uint32_t ins, ord, rs1, rs2;
-    uint32_t n_ins, n_ord, n_rs1, n_rs2;
-    uint32_t pk;  // raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18
+    uint32_t ord, rs1, rs2;        // step i (reloaded from LDS in every phase)
+    uint32_t n_ord, n_rs1, n_rs2;  // step i+1
+    uint32_t m_ord, m_rs1, m_rs2;  // step i+2 (loads in flight)
+    uint32_t fp;                   // OP-FP bit of steps i, i+1, i+2 (bits 0, 1, 2)
+    uint32_t pk;                   // step i: raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18
+    uint32_t g1, g2;               // step i+1 operand entries of sym (loads in flight)
 };
 
 DEV RaStep ra0_step(const Ra0Fn &f) {
     const uint32_t fl = f.pk >> 18;
-    return ra_eval_raw(f.ins, f.ord, f.rs1, f.rs2, f.pk & 0xFFu, (f.pk >> 8) & 0xFFu,
+    return ra_eval_raw((f.fp & 1u) ? 0x53u : 0u, f.ord, f.rs1, f.rs2, f.pk & 0xFFu, (f.pk >> 8) & 0xFFu,
                        ((f.pk >> 16) & 1u) || (fl & 1u), ((f.pk >> 17) & 1u) || (fl & 2u), f.life);
+}
+
+DEV void ra0_fields(const Dev &d, const Ra0Fn &f, uint32_t step, uint32_t &fpbit, uint32_t &ord, uint32_t &rs1,
+                    uint32_t &rs2) {
+    fpbit = ord = rs1 = rs2 = 0;
+    if ((int32_t)step < (int32_t)f.size) {
+        const uint4 w = d.w0_f4[f.start + step];
+        ord = w.x; rs1 = w.y; rs2 = w.z;
+        fpbit = (w.w & 0x7Fu) == 0x53u ? 1u : 0u;
+    }
+}
+
+// True when step i's commit rewrites entry x (x is then not loaded: the patch
+// after the commit supplies it).
+DEV bool ra0_covered(const Ra0Fn &f, uint32_t x) {
+    return ((int32_t)f.ord > 63 && x == f.ord) || ((int32_t)f.rs1 > 63 && x == f.rs1) ||
+           ((int32_t)f.rs2 > 63 && x == f.rs2);
+}
+
+DEV void ra0_gather(const Dev &d, Ra0Fn &f, uint32_t step) {
+    f.g1 = f.g2 = 0;
+    if ((int32_t)step < (int32_t)f.size) {
+        if (virt_ok(d, f.n_rs1) && !ra0_covered(f, f.n_rs1)) f.g1 = d.sym[f.n_rs1 - 64u];
+        if (virt_ok(d, f.n_rs2) && !ra0_covered(f, f.n_rs2)) f.g2 = d.sym[f.n_rs2 - 64u];
+    }
+}
+
+DEV uint32_t ra0_pack(uint32_t g1, uint32_t g2) {
+    return (g1 & 0xFFu) | ((g2 & 0xFFu) << 8) | (((g1 >> 8) & 1u) << 16) | (((g2 >> 8) & 1u) << 17);
+}
+
+// Entry x of a pending gather as this step's commit leaves it (destination,
+// then rs1, then rs2, each with a cleared swapped flag).
+DEV void ra0_patch(uint32_t x, uint32_t &g, const RaStep &r) {
+    if ((int32_t)r.ord > 63 && x == r.ord) g = r.rdr;
+    if ((int32_t)r.rs1 > 63 && x == r.rs1) g = r.r1r;
+    if ((int32_t)r.rs2 > 63 && x == r.rs2) g = r.r2r;
+}
+
+// The commit of ra_commit_syms, storing only entries whose value changes:
+// old1/old2 are the operands' entries before the commit; a destination entry is
+// always stored unless an operand names the same entry (the later store wins).
+DEV void ra0_commit(const Dev &d, const RaStep &r, uint32_t old1, uint32_t old2) {
+    const bool vo = (int32_t)r.ord > 63, v1 = (int32_t)r.rs1 > 63, v2 = (int32_t)r.rs2 > 63;
+    if ((vo && r.ord - 64u >= d.W0) || (v1 && r.rs1 - 64u >= d.W0) || (v2 && r.rs2 - 64u >= d.W0)) {
+        atomicOr(d.flags, EF_OOB);
+        return;
+    }
+    if (vo && !(v1 && r.rs1 == r.ord) && !(v2 && r.rs2 == r.ord)) d.sym[r.ord - 64u] = (uint16_t)r.rdr;
+    if (v1 && !(v2 && r.rs2 == r.rs1) && r.r1r != old1) d.sym[r.rs1 - 64u] = (uint16_t)r.r1r;
+    if (v2 && r.r2r != old2) d.sym[r.rs2 - 64u] = (uint16_t)r.r2r;
 }
 
 DEV void ot_insert(int32_t *key, uint32_t *bits, int32_t o, uint32_t b) {
@@ -959,176 +1076,268 @@ DEV void ot_insert(int32_t *key, uint32_t *bits, int32_t o, uint32_t b) {
     }
 }
 
-DEV uint32_t ot_lookup(const int32_t *key, const uint32_t *bits, int32_t o) {
+DEV int ot_find(const int32_t *key, int32_t o) {
     uint32_t h = (uint32_t)o & 127u;
     for (int k = 0; k < 128; ++k) {
         int32_t kk = key[h];
-        if (kk == o) return bits[h];
-        if (kk == -1) return 0;
+        if (kk == o) return (int)h;
+        if (kk == -1) return -1;
         h = (h + 1) & 127u;
     }
-    return 0;
+    return -1;
 }
 
-constexpr uint32_t RA0_LDS_FNS = 8192;  // per-function fields kept in LDS up to this many lane-0 functions
+DEV uint32_t ot_lookup(const int32_t *key, const uint32_t *bits, int32_t o) {
+    const int h = ot_find(key, o);
+    return h < 0 ? 0u : bits[h];
+}
+
+constexpr uint32_t RA0_LDS_FNS = 8192;  // lane-0 functions whose current fields fit in LDS
+
+DEV void ra0_cur(Ra0Fn &f, const uint32_t *crd, const uint32_t *crs1, const uint32_t *crs2, uint32_t j) {
+    f.ord = crd[j]; f.rs1 = crs1[j]; f.rs2 = crs2[j];
+}
 
 template <int PER>
 __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, const uint32_t *gmin,
-                                                   const uint32_t *gmax, uint32_t *gcur /* 3*L */) {
-    __shared__ uint32_t s_cur[3 * RA0_LDS_FNS];
-    __shared__ int32_t Rv[64], Ro[64], Sv[64], So[64];
-    __shared__ uint32_t Lmin[64], Lmax[64], Bmin[64], Bmax[64], fbits[64], wkey[64];
-    __shared__ int32_t ot_key[128];
-    __shared__ uint32_t ot_bits[128];
+                                                   const uint32_t *gmax, const uint32_t *perm) {
+    __shared__ uint32_t cur_rd[RA0_LDS_FNS], cur_rs1[RA0_LDS_FNS], cur_rs2[RA0_LDS_FNS];
+    __shared__ uint16_t slot_of[RA0_LDS_FNS];                 // group -> slot
+    __shared__ uint32_t mkw[2][RA0_LDS_FNS / 64 / 32];        // waves of slots with posted marks
+    __shared__ int32_t Sv[2][64], So[2][64];                  // region-0 snapshot of step i / i+1
+    __shared__ uint32_t Lmin[2][64], Lmax[2][64], Bmin[2][64], Bmax[2][64], wkey[2][64];
+    __shared__ int32_t ot_key[2][128], mk_key[2][128];        // owner -> flip bits / marked snapshot registers
+    __shared__ uint32_t ot_bits[2][128], mk_lo[2][128], mk_hi[2][128];
     __shared__ int changed;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const uint32_t L = (d.F + 7u) / 8u;
-    uint32_t *cur = L <= RA0_LDS_FNS ? s_cur : gcur;
-    uint32_t *cur_rd = cur, *cur_rs1 = cur + L, *cur_rs2 = cur + 2 * L;
+    const uint32_t cb = nth > 64 ? 64u : 0u;  // first thread of the table-clearing threads
     Ra0Fn fn[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const uint32_t j = tid + q * nth;
-        fn[q].start = 0; fn[q].size = 0; fn[q].life = 0x1Full; fn[q].pk = 0;
-        fn[q].n_ins = fn[q].n_ord = fn[q].n_rs1 = fn[q].n_rs2 = 0;
-        if (j < L) {
-            fn[q].start = d.fstarts[8u * j];
-            fn[q].size = d.fsizes[8u * j];
-            if ((int32_t)fn[q].size > 0) {
-                const uint32_t off = fn[q].start;
-                fn[q].n_ins = d.w0_instr[off]; fn[q].n_ord = d.w0_rd[off];
-                fn[q].n_rs1 = d.w0_rs1[off]; fn[q].n_rs2 = d.w0_rs2[off];
-            }
+        const uint32_t t = tid + q * nth;
+        Ra0Fn &f = fn[q];
+        f.j = 0; f.start = 0; f.size = 0; f.life = 0x1Full; f.pk = 0;
+        f.ord = f.rs1 = f.rs2 = 0;
+        if (t < L) {
+            f.j = perm[t];
+            if (f.j < L) { f.start = d.fstarts[8u * f.j]; f.size = d.fsizes[8u * f.j]; slot_of[f.j] = (uint16_t)t; }
+            else { atomicOr(d.flags, EF_OOB); f.j = 0; }
+        }
+        const uint32_t j = f.j;
+        uint32_t b0, b1, b2;
+        ra0_fields(d, f, 0, b0, f.n_ord, f.n_rs1, f.n_rs2);
+        ra0_gather(d, f, 0);
+        f.pk = ra0_pack(f.g1, f.g2);
+        f.ord = f.n_ord; f.rs1 = f.n_rs1; f.rs2 = f.n_rs2;
+        ra0_fields(d, f, 1, b1, f.n_ord, f.n_rs1, f.n_rs2);
+        ra0_fields(d, f, 2, b2, f.m_ord, f.m_rs1, f.m_rs2);
+        f.fp = b0 | (b1 << 1) | (b2 << 2);
+        if (t < L) {
+            const bool act = (int32_t)f.size > 0;
+            cur_rd[j] = act ? f.ord : 0u; cur_rs1[j] = act ? f.rs1 : 0u; cur_rs2[j] = act ? f.rs2 : 0u;
         }
     }
-    uint32_t nmin = 0xFFFFFFFFu, nmax = 0;
-    if (tid < 64) { Rv[tid] = -1; Ro[tid] = -1; if (maxsize) { nmin = gmin[tid]; nmax = gmax[tid]; } }
+    uint32_t qmin = 0xFFFFFFFFu, qmax = 0;  // wave 0: group lookups of step i+1
+    if (tid < 64) {
+        Sv[0][tid] = -1; So[0][tid] = -1; wkey[0][tid] = 0;
+        const uint32_t a = maxsize ? gmin[tid] : 0xFFFFFFFFu, b = maxsize ? gmax[tid] : 0u;
+        Lmin[0][tid] = a; Bmin[0][tid] = a; Lmax[0][tid] = b; Bmax[0][tid] = b;
+        if (maxsize > 1) { qmin = gmin[64u + tid]; qmax = gmax[64u + tid]; }
+    }
+    for (uint32_t k = tid; k < 128; k += nth) {
+        ot_key[0][k] = -1; ot_bits[0][k] = 0; mk_key[0][k] = -1; mk_lo[0][k] = 0; mk_hi[0][k] = 0;
+        if (k < RA0_LDS_FNS / 64 / 32) mkw[0][k] = 0;
+    }
     __syncthreads();
+    uint32_t fb = 0;   // wave 0: flip bits of snapshot register tid
     for (uint32_t i = 0; i < maxsize; ++i) {
+        const uint32_t c = i & 1u, n = c ^ 1u;
+        // P1: evaluate with no flips; record lookups; start step i+1's gathers
         if (tid < 64) {
-            Sv[tid] = Rv[tid]; So[tid] = Ro[tid]; fbits[tid] = 0; wkey[tid] = 0;
-            Bmin[tid] = nmin; Bmax[tid] = nmax; Lmin[tid] = nmin; Lmax[tid] = nmax;
-            if (i + 1 < maxsize) { nmin = gmin[(uint64_t)(i + 1) * 64u + tid]; nmax = gmax[(uint64_t)(i + 1) * 64u + tid]; }
+            Lmin[n][tid] = qmin; Bmin[n][tid] = qmin; Lmax[n][tid] = qmax; Bmax[n][tid] = qmax;
+            qmin = 0xFFFFFFFFu; qmax = 0;
+            if (i + 2 < maxsize) { qmin = gmin[(uint64_t)(i + 2) * 64u + tid]; qmax = gmax[(uint64_t)(i + 2) * 64u + tid]; }
         }
-        if (tid == 0) changed = 0;
-        // this step's fields (prefetched), the operands' state, next step's prefetch
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const uint32_t j = tid + q * nth;
-            const bool act = j < L && (int32_t)i < (int32_t)fn[q].size;
-            fn[q].ins = fn[q].n_ins; fn[q].ord = fn[q].n_ord; fn[q].rs1 = fn[q].n_rs1; fn[q].rs2 = fn[q].n_rs2;
-            if (j < L) {
-                cur_rd[j] = act ? fn[q].ord : 0u; cur_rs1[j] = act ? fn[q].rs1 : 0u; cur_rs2[j] = act ? fn[q].rs2 : 0u;
-            }
-            if (act) {
-                const bool v1 = virt_ok(d, fn[q].rs1), v2 = virt_ok(d, fn[q].rs2);
-                const uint32_t raw1 = v1 ? d.sym_reg[fn[q].rs1 - 64u] : 0u;
-                const uint32_t raw2 = v2 ? d.sym_reg[fn[q].rs2 - 64u] : 0u;
-                const uint32_t s1 = (v1 && d.sym_sw[fn[q].rs1 - 64u]) ? 1u : 0u;
-                const uint32_t s2 = (v2 && d.sym_sw[fn[q].rs2 - 64u]) ? 1u : 0u;
-                fn[q].pk = raw1 | (raw2 << 8) | (s1 << 16) | (s2 << 17);
-                if ((int32_t)(i + 1) < (int32_t)fn[q].size) {
-                    const uint32_t off = fn[q].start + i + 1;
-                    fn[q].n_ins = d.w0_instr[off]; fn[q].n_ord = d.w0_rd[off];
-                    fn[q].n_rs1 = d.w0_rs1[off]; fn[q].n_rs2 = d.w0_rs2[off];
-                }
+            Ra0Fn &f = fn[q];
+            const uint32_t j = f.j;
+            if ((int32_t)i < (int32_t)f.size) {
+                ra0_cur(f, cur_rd, cur_rs1, cur_rs2, j);
+                const RaStep r = ra0_step(f);
+                if (r.swd) { atomicMin(&Lmin[c][r.rdr], j); atomicMax(&Lmax[c][r.rdr], j + 1u); }
+                if (r.sw1) { atomicMin(&Lmin[c][r.r1r], j); atomicMax(&Lmax[c][r.r1r], j + 1u); }
+                if (r.sw2) { atomicMin(&Lmin[c][r.r2r], j); atomicMax(&Lmax[c][r.r2r], j + 1u); }
+                xref_check_fields(d, 8u * j, f.start, f.size, f.ord, f.rs1, f.rs2);
+                ra0_gather(d, f, i + 1);
             }
         }
-        __syncthreads();
+        lds_barrier();
+        // P2: an owner of a snapshot entry reading it as an operand sees the
+        // swap when a lower group looked the register up in this step
+        if (tid < 64) {
+            uint32_t nb = 0;
+            const int32_t o = So[c][tid], v = Sv[c][tid];
+            if (v >= 64 && o >= 0 && Lmin[c][tid] < (uint32_t)o) {
+                if (cur_rs1[o] == (uint32_t)v) nb |= 1u;
+                if (cur_rs2[o] == (uint32_t)v) nb |= 2u;
+            }
+            if (nb) ot_insert(ot_key[c], ot_bits[c], o, nb);
+            fb = nb;
+            const uint64_t any = __ballot(nb != 0);
+            if (tid == 0) changed = any != 0;
+        }
+        if (tid >= cb)
+            for (uint32_t k = tid - cb; k < 128; k += nth - cb) {
+                ot_key[n][k] = -1; ot_bits[n][k] = 0; mk_key[n][k] = -1; mk_lo[n][k] = 0; mk_hi[n][k] = 0;
+                if (k < 64) wkey[n][k] = 0;
+                if (k < RA0_LDS_FNS / 64 / 32) mkw[n][k] = 0;
+            }
+        lds_barrier();
+        if (changed) {
+            // settle intra-step flips: re-evaluate until the flip bits are stable
+            // (lower groups settle first, so this reaches the in-order result)
+            for (uint32_t round = 0;; ++round) {
+                if (tid < 64) { Lmin[c][tid] = Bmin[c][tid]; Lmax[c][tid] = Bmax[c][tid]; }
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const uint32_t j = tid + q * nth;
-            if (j < L && (int32_t)i < (int32_t)fn[q].size) {
-                const RaStep r = ra0_step(fn[q]);
-                if (r.swd) { atomicMin(&Lmin[r.rdr], j); atomicMax(&Lmax[r.rdr], j + 1u); }
-                if (r.sw1) { atomicMin(&Lmin[r.r1r], j); atomicMax(&Lmax[r.r1r], j + 1u); }
-                if (r.sw2) { atomicMin(&Lmin[r.r2r], j); atomicMax(&Lmax[r.r2r], j + 1u); }
+                for (int q = 0; q < PER; ++q)
+                    if ((int32_t)i < (int32_t)fn[q].size)
+                        fn[q].pk = (fn[q].pk & 0x3FFFFu) | (ot_lookup(ot_key[c], ot_bits[c], (int32_t)fn[q].j) << 18);
+                lds_barrier();
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    const uint32_t j = fn[q].j;
+                    if ((int32_t)i < (int32_t)fn[q].size) {
+                        ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
+                        const RaStep r = ra0_step(fn[q]);
+                        if (r.swd) { atomicMin(&Lmin[c][r.rdr], j); atomicMax(&Lmax[c][r.rdr], j + 1u); }
+                        if (r.sw1) { atomicMin(&Lmin[c][r.r1r], j); atomicMax(&Lmax[c][r.r1r], j + 1u); }
+                        if (r.sw2) { atomicMin(&Lmin[c][r.r2r], j); atomicMax(&Lmax[c][r.r2r], j + 1u); }
+                    }
+                }
+                if (tid >= cb)
+                    for (uint32_t k = tid - cb; k < 128; k += nth - cb) { ot_key[c][k] = -1; ot_bits[c][k] = 0; }
+                lds_barrier();
+                if (tid < 64) {
+                    uint32_t nb = 0;
+                    const int32_t o = So[c][tid], v = Sv[c][tid];
+                    if (v >= 64 && o >= 0 && Lmin[c][tid] < (uint32_t)o) {
+                        if (cur_rs1[o] == (uint32_t)v) nb |= 1u;
+                        if (cur_rs2[o] == (uint32_t)v) nb |= 2u;
+                    }
+                    if (nb) ot_insert(ot_key[c], ot_bits[c], o, nb);
+                    const uint64_t diff = __ballot(nb != fb);
+                    fb = nb;
+                    if (tid == 0) changed = diff != 0;
+                }
+                lds_barrier();
+                if (!changed) break;
+                if (round > L + 1) { if (tid == 0) atomicOr(d.flags, EF_OOB); break; }
             }
         }
-        __syncthreads();
-        for (uint32_t round = 0;; ++round) {
-            if (tid < 128) { ot_key[tid] = -1; ot_bits[tid] = 0; }
-            __syncthreads();
-            if (tid < 64) {
-                uint32_t nb = 0;
-                const int32_t o = So[tid];
-                if (Sv[tid] >= 64 && o >= 0 && Lmin[tid] < (uint32_t)o) {
-                    if (cur_rs1[o] == (uint32_t)Sv[tid]) nb |= 1u;
-                    if (cur_rs2[o] == (uint32_t)Sv[tid]) nb |= 2u;
-                }
-                if (nb) ot_insert(ot_key, ot_bits, o, nb);
-                if (nb != fbits[tid]) { fbits[tid] = nb; changed = 1; }
-            }
-            __syncthreads();
-            int again = changed;
-            if (again && round > L + 1) { again = 0; if (tid == 0) atomicOr(d.flags, EF_OOB); }
-            if (!again) break;
-            __syncthreads();
-            if (tid < 64) { Lmin[tid] = Bmin[tid]; Lmax[tid] = Bmax[tid]; }
-            if (tid == 0) changed = 0;
-#pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const uint32_t j = tid + q * nth;
-                if (j < L && (int32_t)i < (int32_t)fn[q].size) {
-                    const uint32_t f = ot_lookup(ot_key, ot_bits, (int32_t)j);
-                    fn[q].pk = (fn[q].pk & 0x3FFFFu) | (f << 18);
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < PER; ++q) {
-                const uint32_t j = tid + q * nth;
-                if (j < L && (int32_t)i < (int32_t)fn[q].size) {
-                    const RaStep r = ra0_step(fn[q]);
-                    if (r.swd) { atomicMin(&Lmin[r.rdr], j); atomicMax(&Lmax[r.rdr], j + 1u); }
-                    if (r.sw1) { atomicMin(&Lmin[r.r1r], j); atomicMax(&Lmax[r.r1r], j + 1u); }
-                    if (r.sw2) { atomicMin(&Lmin[r.r2r], j); atomicMax(&Lmax[r.r2r], j + 1u); }
-                }
-            }
-            __syncthreads();
-        }
-        // commit: lifetime, region-0 write keys (group order: the last writer wins)
+        // P3: commit lifetimes; region-0 write keys (the last group wins);
+        // marks on the snapshot's virtual registers, posted to their owners
         uint32_t rr[PER];  // committed r1r | r2r << 8 | rdr << 16
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const uint32_t j = tid + q * nth;
+            const uint32_t j = fn[q].j;
             rr[q] = 0;
-            if (j < L && (int32_t)i < (int32_t)fn[q].size) {
+            if ((int32_t)i < (int32_t)fn[q].size) {
+                ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
                 const RaStep r = ra0_step(fn[q]);
                 rr[q] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
                 fn[q].life = r.newlife;
-                if (r.ord) atomicMax(&wkey[r.rdr], ((j + 1u) << 2) | 1u);
-                if (r.r1r != r.rdr) atomicMax(&wkey[r.r1r], ((j + 1u) << 2) | 2u);
-                if (r.r2r != r.rdr) atomicMax(&wkey[r.r2r], ((j + 1u) << 2) | 3u);
+                if (r.ord) atomicMax(&wkey[c][r.rdr], ((j + 1u) << 2) | 1u);
+                if (r.r1r != r.rdr) atomicMax(&wkey[c][r.r1r], ((j + 1u) << 2) | 2u);
+                if (r.r2r != r.rdr) atomicMax(&wkey[c][r.r2r], ((j + 1u) << 2) | 3u);
             }
         }
-        __syncthreads();
+        if (tid < 64) {
+            const int32_t v = Sv[c][tid], o = So[c][tid];
+            Sv[n][tid] = v; So[n][tid] = o;
+            if (v >= 64 && o >= 0 && Lmax[c][tid] != 0) {
+                bool mark = true;
+                const uint32_t uv = (uint32_t)v;
+                if (cur_rd[o] == uv || cur_rs1[o] == uv || cur_rs2[o] == uv) mark = Lmax[c][tid] > (uint32_t)o + 1u;
+                if (mark) {
+                    uint32_t h = (uint32_t)o & 127u;
+                    for (int k = 0; k < 128; ++k) {
+                        const int32_t prev = atomicCAS(&mk_key[c][h], -1, o);
+                        if (prev == -1 || prev == o) break;
+                        h = (h + 1) & 127u;
+                    }
+                    if (tid < 32) atomicOr(&mk_lo[c][h], 1u << tid); else atomicOr(&mk_hi[c][h], 1u << (tid - 32));
+                    const uint32_t ws = (uint32_t)slot_of[o] >> 6;
+                    atomicOr(&mkw[c][ws >> 5], 1u << (ws & 31u));
+                }
+            }
+        }
+        lds_barrier();
+        // P4: snapshot of step i+1, commit of the symbol state, marks, and the
+        // shift to step i+1 (its operand state patched with what this step wrote)
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const uint32_t j = tid + q * nth;
-            if (j < L && (int32_t)i < (int32_t)fn[q].size) {
+            const uint32_t t = tid + q * nth;
+            Ra0Fn &f = fn[q];
+            const uint32_t j = f.j;
+            if (t >= L) continue;
+            const bool act = (int32_t)i < (int32_t)f.size;
+            if (act) {
+                ra0_cur(f, cur_rd, cur_rs1, cur_rs2, j);
                 RaStep r;
-                r.ord = fn[q].ord; r.rs1 = fn[q].rs1; r.rs2 = fn[q].rs2;
+                r.ord = f.ord; r.rs1 = f.rs1; r.rs2 = f.rs2;
                 r.r1r = rr[q] & 0xFFu; r.r2r = (rr[q] >> 8) & 0xFFu; r.rdr = (rr[q] >> 16) & 0xFFu;
-                if (r.ord && wkey[r.rdr] == (((j + 1u) << 2) | 1u)) { Rv[r.rdr] = (int32_t)r.ord; Ro[r.rdr] = (int32_t)j; }
-                if (r.r1r != r.rdr && wkey[r.r1r] == (((j + 1u) << 2) | 2u)) { Rv[r.r1r] = -1; Ro[r.r1r] = -1; }
-                if (r.r2r != r.rdr && wkey[r.r2r] == (((j + 1u) << 2) | 3u)) { Rv[r.r2r] = -1; Ro[r.r2r] = -1; }
-                ra_commit_syms(d, r, true);
+                if (r.ord && wkey[c][r.rdr] == (((j + 1u) << 2) | 1u)) { Sv[n][r.rdr] = (int32_t)r.ord; So[n][r.rdr] = (int32_t)j; }
+                if (r.r1r != r.rdr && wkey[c][r.r1r] == (((j + 1u) << 2) | 2u)) { Sv[n][r.r1r] = -1; So[n][r.r1r] = -1; }
+                if (r.r2r != r.rdr && wkey[c][r.r2r] == (((j + 1u) << 2) | 3u)) { Sv[n][r.r2r] = -1; So[n][r.r2r] = -1; }
+                ra0_commit(d, r, (f.pk & 0xFFu) | (((f.pk >> 16) & 1u) << 8), ((f.pk >> 8) & 0xFFu) | (((f.pk >> 17) & 1u) << 8));
+                ra0_patch(f.n_rs1, f.g1, r);
+                ra0_patch(f.n_rs2, f.g2, r);
+            }
+            const uint32_t ws = t >> 6;
+            const int h = ((mkw[c][ws >> 5] >> (ws & 31u)) & 1u) ? ot_find(mk_key[c], (int32_t)j) : -1;
+            if (h >= 0) {
+                uint64_t m = ((uint64_t)mk_hi[c][h] << 32) | mk_lo[c][h];
+                while (m) {
+                    const uint32_t reg = (uint32_t)__builtin_ctzll(m);
+                    m &= m - 1;
+                    const uint32_t v = (uint32_t)Sv[c][reg];
+                    if (v - 64u < d.W0) sym_mark(d, v - 64u);
+                    else atomicOr(d.flags, EF_OOB);
+                    if (f.n_rs1 == v) f.g1 |= 0x100u;
+                    if (f.n_rs2 == v) f.g2 |= 0x100u;
+                }
+            }
+            if (act) {
+                f.pk = ra0_pack(f.g1, f.g2);
+                const bool nx = (int32_t)(i + 1) < (int32_t)f.size;
+                cur_rd[j] = nx ? f.n_ord : 0u; cur_rs1[j] = nx ? f.n_rs1 : 0u; cur_rs2[j] = nx ? f.n_rs2 : 0u;
+                f.n_ord = f.m_ord; f.n_rs1 = f.m_rs1; f.n_rs2 = f.m_rs2;
+                uint32_t b3;
+                ra0_fields(d, f, i + 3, b3, f.m_ord, f.m_rs1, f.m_rs2);
+                f.fp = (f.fp >> 1) | (b3 << 2);
             }
         }
-        __syncthreads();
-        // marks on the snapshot's virtual registers
-        if (tid < 64 && Sv[tid] >= 64 && Lmax[tid] != 0) {
-            const uint32_t v = (uint32_t)Sv[tid];
-            bool mark = true;
-            const int32_t o = So[tid];
-            if (o >= 0 && (cur_rd[o] == v || cur_rs1[o] == v || cur_rs2[o] == v)) mark = Lmax[tid] > (uint32_t)o + 1u;
-            if (mark) {
-                if (v - 64u < d.W0) d.sym_sw[v - 64u] = 1;
-                else atomicOr(d.flags, EF_OOB);
-            }
-        }
-        __syncthreads();
+    }
+}
+
+// Every word naming a register that a function outside the register's own
+// function also names must belong to that same function (see xref_note).
+__global__ __launch_bounds__(256) void k_xref_check(Dev d) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= d.W0) return;
+    const uint4 w = d.w0_f4[i];
+    const uint32_t vs[3] = {w.x, w.y, w.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t v = vs[k];
+        if (!((int32_t)v > 63) || v - 64u >= d.W0) continue;
+        const uint32_t t = d.xref[v - 64u];
+        if (!t) continue;
+        uint32_t lo = 0, hi = d.F;  // function of word i: last start <= i
+        while (hi - lo > 1) { uint32_t m = (lo + hi) / 2; if (d.fstarts[m] <= i) lo = m; else hi = m; }
+        if (lo + 1u != t) atomicOr(d.flags, EF_XCONFLICT);
     }
 }
 
@@ -1136,7 +1345,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, cons
 // per swapped operand and one store for a swapped destination.
 struct RaCountOp {
     Dev d;
-    DEV uint32_t sw(uint32_t r) const { return ((int32_t)r > 63 && r - 64u < d.W0 && d.sym_sw[r - 64u]) ? 1u : 0u; }
+    DEV uint32_t sw(uint32_t r) const { return ((int32_t)r > 63 && r - 64u < d.W0 && sym_swapped(d, r - 64u)) ? 1u : 0u; }
     DEV uint32_t value(uint64_t i) const {
         if (!d.usedi[i]) return 0;
         return 1u + sw(d.w0_rs1[i]) + sw(d.w0_rs2[i]) + sw(d.w0_rd[i]);
@@ -1148,7 +1357,7 @@ struct RaCountOp {
 DEV uint32_t spill_of(const Dev &d, uint32_t v) {
     uint32_t lo = 0, hi = d.F;  // last start <= v
     while (hi - lo > 1) { uint32_t m = (lo + hi) / 2; if (d.fstarts[m] <= v) lo = m; else hi = m; }
-    return d.sym_sw[d.fstarts[lo]] ? 1u : 0u;
+    return sym_swapped(d, d.fstarts[lo]) ? 1u : 0u;
 }
 
 // Emit (:1552-1697).
@@ -1163,23 +1372,26 @@ __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
     if (((int32_t)r1 > 63 && r1 - 64u >= d.W0) || ((int32_t)r2 > 63 && r2 - 64u >= d.W0) ||
         ((int32_t)rdv > 63 && rdv - 64u >= d.W0)) { atomicOr(d.flags, EF_OOB); return; }
     if ((int32_t)r1 > 63) {
-        bool sw = d.sym_sw[r1 - 64u];
+        const uint32_t e = d.sym[r1 - 64u];
+        const bool sw = (e >> 8) != 0;
         l1 = sw ? base : -1;
         st1 = spill_of(d, r1 - 64u);
-        a1 = sw ? (((ins == 0xC0007053u) || (ins & 0x7Fu) == 0x53u) ? 37u : 5u) : d.sym_reg[r1 - 64u];
+        a1 = sw ? (((ins == 0xC0007053u) || (ins & 0x7Fu) == 0x53u) ? 37u : 5u) : (e & 0xFFu);
     }
     if ((int32_t)r2 > 63) {
-        bool sw = d.sym_sw[r2 - 64u];
+        const uint32_t e = d.sym[r2 - 64u];
+        const bool sw = (e >> 8) != 0;
         l2 = sw ? base : -1;  // the reference tests its own, still -1, offset (D6)
         st2 = spill_of(d, r2 - 64u);
-        a2 = sw ? (((ins & 0x7Fu) == 0x53u) ? 38u : 6u) : d.sym_reg[r2 - 64u];
+        a2 = sw ? (((ins & 0x7Fu) == 0x53u) ? 38u : 6u) : (e & 0xFFu);
     }
     const int32_t mo = base + (l1 > 0 ? 1 : 0) + (l2 > 0 ? 1 : 0);
     if ((int32_t)rdv > 63) {
-        bool sw = d.sym_sw[rdv - 64u];
+        const uint32_t e = d.sym[rdv - 64u];
+        const bool sw = (e >> 8) != 0;
         rdo = sw ? mo + 1 : -1;
         std_ = spill_of(d, rdv - 64u);
-        ad = d.sym_reg[rdv - 64u];
+        ad = e & 0xFFu;
     }
     if ((rdo >= 0 && (uint32_t)rdo >= d.W1) || (uint32_t)mo >= d.W1) { atomicOr(d.flags, EF_OOB); return; }
     if (l1 >= 0) { d.w1_instr[l1] = (((st1 - 1u) * 4u) << 20) | 0x2003u; d.w1_rd[l1] = 5; }
