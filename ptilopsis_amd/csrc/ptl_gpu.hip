@@ -986,71 +986,58 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
 // t = tid + q*blockDim; slot t runs group perm[t], the groups ordered by
 // function size, largest first, so that the functions still running at a step
 // fill the first waves and finished waves skip each phase) and keeps their
-// state in registers: the fields of steps i+1 and
-// i+2 (the latter in flight), the operand state of step i (packed) and of step
-// i+1 (in flight, patched with step i's commit and marks when it lands).  The
-// fields of step i sit in LDS (cur_*), where the snapshot owners' checks read
-// them too.
-// A function's virtual registers are read and written only by its own thread
-// (marks by other groups are posted to the owner through LDS), so the three
-// barriers per step order LDS alone and never wait for HBM.  Per-step LDS
-// tables are double-buffered by step parity so that no barrier is needed
-// between the end of a step and the start of the next.  Of an instruction
-// word, register allocation looks only at whether its major opcode is OP-FP
-// (0x53): the other word tests of ra_eval_raw (0xC0007053, 0xD0007053) are
-// OP-FP words themselves, so the lockstep keeps that bit alone.
+// state in registers: the fields of step i+1 and the packed operand state of
+// step i.  The fields of step i sit in LDS (cur_*), where the snapshot checks
+// read them too.  A function's virtual registers are read and written only by
+// its own thread (marks by other groups reach the owner through LDS), so the
+// barriers of a step order LDS alone.  Each step ends with one batch of loads
+// for the next (operand state and fields of all functions of the thread) and a
+// single wait for it.  Per-step LDS tables are double-buffered by step parity
+// so that no barrier is needed between the end of a step and the start of the
+// next.
+//
+// Snapshot owners: only a group owning an entry of the region-0 snapshot (a
+// virtual register it wrote there) can see a swap made earlier in the same
+// step, so only owners take part in the in-order settle; every other group's
+// step result is final after its first evaluation.  Each function keeps the
+// set of snapshot registers it may own (`own`, a superset refreshed against
+// the snapshot every step), so owners find their flips and marks by register
+// without any table.
+//
+// Of an instruction word, register allocation looks only at whether its major
+// opcode is OP-FP (0x53): the other word tests of ra_eval_raw (0xC0007053,
+// 0xD0007053) are OP-FP words themselves, so the lockstep keeps that bit alone.
 struct Ra0Fn {
-    uint32_t j, start, size;       // group, first word, word count
+    uint32_t j, start, size;              // group, first word, word count
     uint64_t life;
-    uint32_t ord, rs1, rs2;        // step i (reloaded from LDS in every phase)
-    uint32_t n_ord, n_rs1, n_rs2;  // step i+1
-    uint32_t m_ord, m_rs1, m_rs2;  // step i+2 (loads in flight)
-    uint32_t fp;                   // OP-FP bit of steps i, i+1, i+2 (bits 0, 1, 2)
-    uint32_t pk;                   // step i: raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18
-    uint32_t g1, g2;               // step i+1 operand entries of sym (loads in flight)
+    uint64_t own;                         // snapshot registers this function may own
+    uint32_t ord, rs1, rs2;               // step i (reloaded from LDS in every phase)
+    uint4 nw;                             // step i+1: rd, rs1, rs2, instruction (zeros past the end)
+    uint32_t fp;                          // OP-FP bit of step i
+    uint32_t pk;                          // step i: raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18
 };
 
 DEV RaStep ra0_step(const Ra0Fn &f) {
     const uint32_t fl = f.pk >> 18;
-    return ra_eval_raw((f.fp & 1u) ? 0x53u : 0u, f.ord, f.rs1, f.rs2, f.pk & 0xFFu, (f.pk >> 8) & 0xFFu,
+    return ra_eval_raw(f.fp ? 0x53u : 0u, f.ord, f.rs1, f.rs2, f.pk & 0xFFu, (f.pk >> 8) & 0xFFu,
                        ((f.pk >> 16) & 1u) || (fl & 1u), ((f.pk >> 17) & 1u) || (fl & 2u), f.life);
 }
 
-DEV void ra0_fields(const Dev &d, const Ra0Fn &f, uint32_t step, uint32_t &fpbit, uint32_t &ord, uint32_t &rs1,
-                    uint32_t &rs2) {
-    fpbit = ord = rs1 = rs2 = 0;
-    if ((int32_t)step < (int32_t)f.size) {
-        const uint4 w = d.w0_f4[f.start + step];
-        ord = w.x; rs1 = w.y; rs2 = w.z;
-        fpbit = (w.w & 0x7Fu) == 0x53u ? 1u : 0u;
-    }
+// Word `step` of the function (zeros past its end).
+DEV uint4 ra0_word(const Dev &d, const Ra0Fn &f, uint32_t step) {
+    return (int32_t)step < (int32_t)f.size ? d.w0_f4[f.start + step] : make_uint4(0, 0, 0, 0);
 }
 
-// True when step i's commit rewrites entry x (x is then not loaded: the patch
-// after the commit supplies it).
-DEV bool ra0_covered(const Ra0Fn &f, uint32_t x) {
-    return ((int32_t)f.ord > 63 && x == f.ord) || ((int32_t)f.rs1 > 63 && x == f.rs1) ||
-           ((int32_t)f.rs2 > 63 && x == f.rs2);
-}
+DEV uint32_t ra0_fpbit(uint32_t ins) { return (ins & 0x7Fu) == 0x53u ? 1u : 0u; }
 
-DEV void ra0_gather(const Dev &d, Ra0Fn &f, uint32_t step) {
-    f.g1 = f.g2 = 0;
-    if ((int32_t)step < (int32_t)f.size) {
-        if (virt_ok(d, f.n_rs1) && !ra0_covered(f, f.n_rs1)) f.g1 = d.sym[f.n_rs1 - 64u];
-        if (virt_ok(d, f.n_rs2) && !ra0_covered(f, f.n_rs2)) f.g2 = d.sym[f.n_rs2 - 64u];
-    }
+// sym entries of a word's operands (0 for physical operands).
+DEV void ra0_gather(const Dev &d, uint32_t rs1, uint32_t rs2, uint32_t &g1, uint32_t &g2) {
+    g1 = virt_ok(d, rs1) ? d.sym[rs1 - 64u] : 0u;
+    g2 = virt_ok(d, rs2) ? d.sym[rs2 - 64u] : 0u;
 }
 
 DEV uint32_t ra0_pack(uint32_t g1, uint32_t g2) {
     return (g1 & 0xFFu) | ((g2 & 0xFFu) << 8) | (((g1 >> 8) & 1u) << 16) | (((g2 >> 8) & 1u) << 17);
-}
-
-// Entry x of a pending gather as this step's commit leaves it (destination,
-// then rs1, then rs2, each with a cleared swapped flag).
-DEV void ra0_patch(uint32_t x, uint32_t &g, const RaStep &r) {
-    if ((int32_t)r.ord > 63 && x == r.ord) g = r.rdr;
-    if ((int32_t)r.rs1 > 63 && x == r.rs1) g = r.r1r;
-    if ((int32_t)r.rs2 > 63 && x == r.rs2) g = r.r2r;
 }
 
 // The commit of ra_commit_syms, storing only entries whose value changes:
@@ -1067,29 +1054,21 @@ DEV void ra0_commit(const Dev &d, const RaStep &r, uint32_t old1, uint32_t old2)
     if (v2 && r.r2r != old2) d.sym[r.rs2 - 64u] = (uint16_t)r.r2r;
 }
 
-DEV void ot_insert(int32_t *key, uint32_t *bits, int32_t o, uint32_t b) {
-    uint32_t h = (uint32_t)o & 127u;
-    for (int k = 0; k < 128; ++k) {
-        int32_t prev = atomicCAS(&key[h], -1, o);
-        if (prev == -1 || prev == o) { atomicOr(&bits[h], b); return; }
-        h = (h + 1) & 127u;
+// The snapshot registers of `own` that group j owns in snapshot (Sv, So).
+DEV uint64_t ra0_owned(uint64_t own, const int32_t *Sv, const int32_t *So, uint32_t j) {
+    uint64_t o = 0;
+    while (own) {
+        const uint32_t r = (uint32_t)__builtin_ctzll(own);
+        own &= own - 1;
+        if (So[r] == (int32_t)j && Sv[r] >= 64) o |= 1ull << r;
     }
+    return o;
 }
 
-DEV int ot_find(const int32_t *key, int32_t o) {
-    uint32_t h = (uint32_t)o & 127u;
-    for (int k = 0; k < 128; ++k) {
-        int32_t kk = key[h];
-        if (kk == o) return (int)h;
-        if (kk == -1) return -1;
-        h = (h + 1) & 127u;
-    }
-    return -1;
-}
-
-DEV uint32_t ot_lookup(const int32_t *key, const uint32_t *bits, int32_t o) {
-    const int h = ot_find(key, o);
-    return h < 0 ? 0u : bits[h];
+DEV uint32_t ra0_flips(uint64_t o, const uint32_t *flip) {
+    uint32_t f = 0;
+    while (o) { const uint32_t r = (uint32_t)__builtin_ctzll(o); o &= o - 1; f |= flip[r]; }
+    return f;
 }
 
 constexpr uint32_t RA0_LDS_FNS = 8192;  // lane-0 functions whose current fields fit in LDS
@@ -1098,16 +1077,29 @@ DEV void ra0_cur(Ra0Fn &f, const uint32_t *crd, const uint32_t *crs1, const uint
     f.ord = crd[j]; f.rs1 = crs1[j]; f.rs2 = crs2[j];
 }
 
+DEV void ra0_lookups(uint32_t *mn, uint32_t *mx, const RaStep &r, uint32_t j) {
+    if (r.swd) { atomicMin(&mn[r.rdr], j); atomicMax(&mx[r.rdr], j + 1u); }
+    if (r.sw1) { atomicMin(&mn[r.r1r], j); atomicMax(&mx[r.r1r], j + 1u); }
+    if (r.sw2) { atomicMin(&mn[r.r2r], j); atomicMax(&mx[r.r2r], j + 1u); }
+}
+
+DEV void ra0_keys(uint32_t *wkey, const RaStep &r, uint32_t j) {
+    if (r.ord) atomicMax(&wkey[r.rdr], ((j + 1u) << 2) | 1u);
+    if (r.r1r != r.rdr) atomicMax(&wkey[r.r1r], ((j + 1u) << 2) | 2u);
+    if (r.r2r != r.rdr) atomicMax(&wkey[r.r2r], ((j + 1u) << 2) | 3u);
+}
+
 template <int PER>
 __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, const uint32_t *gmin,
                                                    const uint32_t *gmax, const uint32_t *perm) {
     __shared__ uint32_t cur_rd[RA0_LDS_FNS], cur_rs1[RA0_LDS_FNS], cur_rs2[RA0_LDS_FNS];
-    __shared__ uint16_t slot_of[RA0_LDS_FNS];                 // group -> slot
-    __shared__ uint32_t mkw[2][RA0_LDS_FNS / 64 / 32];        // waves of slots with posted marks
+    __shared__ uint32_t rrs[RA0_LDS_FNS];                     // step result per group: r1r | r2r << 8 | rdr << 16
     __shared__ int32_t Sv[2][64], So[2][64];                  // region-0 snapshot of step i / i+1
-    __shared__ uint32_t Lmin[2][64], Lmax[2][64], Bmin[2][64], Bmax[2][64], wkey[2][64];
-    __shared__ int32_t ot_key[2][128], mk_key[2][128];        // owner -> flip bits / marked snapshot registers
-    __shared__ uint32_t ot_bits[2][128], mk_lo[2][128], mk_hi[2][128];
+    __shared__ uint32_t Lmin[2][64], Lmax[2][64];             // lookups by all groups but the owners
+    __shared__ uint32_t Omin[2][64], Omax[2][64];             // lookups by the (possible) owners
+    __shared__ uint32_t wkey[2][64];
+    __shared__ uint32_t flip[64];                             // flip bits per snapshot entry
+    __shared__ uint32_t mk[2][2];                             // snapshot entries marked at step i
     __shared__ int changed;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const uint32_t L = (d.F + 7u) / 8u;
@@ -1117,60 +1109,54 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, cons
     for (int q = 0; q < PER; ++q) {
         const uint32_t t = tid + q * nth;
         Ra0Fn &f = fn[q];
-        f.j = 0; f.start = 0; f.size = 0; f.life = 0x1Full; f.pk = 0;
-        f.ord = f.rs1 = f.rs2 = 0;
+        f.j = 0; f.start = 0; f.size = 0; f.life = 0x1Full; f.own = 0;
         if (t < L) {
             f.j = perm[t];
-            if (f.j < L) { f.start = d.fstarts[8u * f.j]; f.size = d.fsizes[8u * f.j]; slot_of[f.j] = (uint16_t)t; }
+            if (f.j < L) { f.start = d.fstarts[8u * f.j]; f.size = d.fsizes[8u * f.j]; }
             else { atomicOr(d.flags, EF_OOB); f.j = 0; }
         }
-        const uint32_t j = f.j;
-        uint32_t b0, b1, b2;
-        ra0_fields(d, f, 0, b0, f.n_ord, f.n_rs1, f.n_rs2);
-        ra0_gather(d, f, 0);
-        f.pk = ra0_pack(f.g1, f.g2);
-        f.ord = f.n_ord; f.rs1 = f.n_rs1; f.rs2 = f.n_rs2;
-        ra0_fields(d, f, 1, b1, f.n_ord, f.n_rs1, f.n_rs2);
-        ra0_fields(d, f, 2, b2, f.m_ord, f.m_rs1, f.m_rs2);
-        f.fp = b0 | (b1 << 1) | (b2 << 2);
+        const uint4 w0 = ra0_word(d, f, 0);
+        f.ord = w0.x; f.rs1 = w0.y; f.rs2 = w0.z; f.fp = ra0_fpbit(w0.w);
+        f.nw = ra0_word(d, f, 1);
+        uint32_t g1 = 0, g2 = 0;
+        if ((int32_t)f.size > 0) ra0_gather(d, f.rs1, f.rs2, g1, g2);
+        f.pk = ra0_pack(g1, g2);
         if (t < L) {
             const bool act = (int32_t)f.size > 0;
-            cur_rd[j] = act ? f.ord : 0u; cur_rs1[j] = act ? f.rs1 : 0u; cur_rs2[j] = act ? f.rs2 : 0u;
+            cur_rd[f.j] = act ? f.ord : 0u; cur_rs1[f.j] = act ? f.rs1 : 0u; cur_rs2[f.j] = act ? f.rs2 : 0u;
         }
     }
     uint32_t qmin = 0xFFFFFFFFu, qmax = 0;  // wave 0: group lookups of step i+1
     if (tid < 64) {
         Sv[0][tid] = -1; So[0][tid] = -1; wkey[0][tid] = 0;
-        const uint32_t a = maxsize ? gmin[tid] : 0xFFFFFFFFu, b = maxsize ? gmax[tid] : 0u;
-        Lmin[0][tid] = a; Bmin[0][tid] = a; Lmax[0][tid] = b; Bmax[0][tid] = b;
+        Lmin[0][tid] = maxsize ? gmin[tid] : 0xFFFFFFFFu; Lmax[0][tid] = maxsize ? gmax[tid] : 0u;
+        Omin[0][tid] = 0xFFFFFFFFu; Omax[0][tid] = 0;
         if (maxsize > 1) { qmin = gmin[64u + tid]; qmax = gmax[64u + tid]; }
-    }
-    for (uint32_t k = tid; k < 128; k += nth) {
-        ot_key[0][k] = -1; ot_bits[0][k] = 0; mk_key[0][k] = -1; mk_lo[0][k] = 0; mk_hi[0][k] = 0;
-        if (k < RA0_LDS_FNS / 64 / 32) mkw[0][k] = 0;
     }
     __syncthreads();
     uint32_t fb = 0;   // wave 0: flip bits of snapshot register tid
     for (uint32_t i = 0; i < maxsize; ++i) {
         const uint32_t c = i & 1u, n = c ^ 1u;
-        // P1: evaluate with no flips; record lookups; start step i+1's gathers
-        if (tid < 64) {
-            Lmin[n][tid] = qmin; Bmin[n][tid] = qmin; Lmax[n][tid] = qmax; Bmax[n][tid] = qmax;
-            qmin = 0xFFFFFFFFu; qmax = 0;
-            if (i + 2 < maxsize) { qmin = gmin[(uint64_t)(i + 2) * 64u + tid]; qmax = gmax[(uint64_t)(i + 2) * 64u + tid]; }
-        }
+        // P1: every group evaluates with no flips; a group that cannot own a
+        // snapshot entry finalizes its step, a possible owner keeps its
+        // lookups apart
+        if (tid < 64) { Lmin[n][tid] = qmin; Lmax[n][tid] = qmax; Omin[n][tid] = 0xFFFFFFFFu; Omax[n][tid] = 0; }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             Ra0Fn &f = fn[q];
             const uint32_t j = f.j;
             if ((int32_t)i < (int32_t)f.size) {
                 ra0_cur(f, cur_rd, cur_rs1, cur_rs2, j);
-                const RaStep r = ra0_step(f);
-                if (r.swd) { atomicMin(&Lmin[c][r.rdr], j); atomicMax(&Lmax[c][r.rdr], j + 1u); }
-                if (r.sw1) { atomicMin(&Lmin[c][r.r1r], j); atomicMax(&Lmax[c][r.r1r], j + 1u); }
-                if (r.sw2) { atomicMin(&Lmin[c][r.r2r], j); atomicMax(&Lmax[c][r.r2r], j + 1u); }
                 xref_check_fields(d, 8u * j, f.start, f.size, f.ord, f.rs1, f.rs2);
-                ra0_gather(d, f, i + 1);
+                const RaStep r = ra0_step(f);
+                if (f.own) {
+                    ra0_lookups(Omin[c], Omax[c], r, j);
+                } else {
+                    ra0_lookups(Lmin[c], Lmax[c], r, j);
+                    rrs[j] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
+                    f.life = r.newlife;
+                    ra0_keys(wkey[c], r, j);
+                }
             }
         }
         lds_barrier();
@@ -1179,54 +1165,47 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, cons
         if (tid < 64) {
             uint32_t nb = 0;
             const int32_t o = So[c][tid], v = Sv[c][tid];
-            if (v >= 64 && o >= 0 && Lmin[c][tid] < (uint32_t)o) {
+            if (v >= 64 && o >= 0 && min(Lmin[c][tid], Omin[c][tid]) < (uint32_t)o) {
                 if (cur_rs1[o] == (uint32_t)v) nb |= 1u;
                 if (cur_rs2[o] == (uint32_t)v) nb |= 2u;
             }
-            if (nb) ot_insert(ot_key[c], ot_bits[c], o, nb);
+            flip[tid] = nb;
             fb = nb;
             const uint64_t any = __ballot(nb != 0);
             if (tid == 0) changed = any != 0;
         }
         if (tid >= cb)
-            for (uint32_t k = tid - cb; k < 128; k += nth - cb) {
-                ot_key[n][k] = -1; ot_bits[n][k] = 0; mk_key[n][k] = -1; mk_lo[n][k] = 0; mk_hi[n][k] = 0;
-                if (k < 64) wkey[n][k] = 0;
-                if (k < RA0_LDS_FNS / 64 / 32) mkw[n][k] = 0;
-            }
+            for (uint32_t k = tid - cb; k < 64; k += nth - cb) wkey[n][k] = 0;
         lds_barrier();
         if (changed) {
-            // settle intra-step flips: re-evaluate until the flip bits are stable
-            // (lower groups settle first, so this reaches the in-order result)
+            // settle intra-step flips among the owners: re-evaluate until the
+            // flip bits are stable (lower groups settle first, so this reaches
+            // the in-order result)
             for (uint32_t round = 0;; ++round) {
-                if (tid < 64) { Lmin[c][tid] = Bmin[c][tid]; Lmax[c][tid] = Bmax[c][tid]; }
+                if (tid < 64) { Omin[c][tid] = 0xFFFFFFFFu; Omax[c][tid] = 0; }
 #pragma unroll
                 for (int q = 0; q < PER; ++q)
-                    if ((int32_t)i < (int32_t)fn[q].size)
-                        fn[q].pk = (fn[q].pk & 0x3FFFFu) | (ot_lookup(ot_key[c], ot_bits[c], (int32_t)fn[q].j) << 18);
+                    if (fn[q].own && (int32_t)i < (int32_t)fn[q].size)
+                        fn[q].pk = (fn[q].pk & 0x3FFFFu) |
+                                   (ra0_flips(ra0_owned(fn[q].own, Sv[c], So[c], fn[q].j), flip) << 18);
                 lds_barrier();
 #pragma unroll
                 for (int q = 0; q < PER; ++q) {
                     const uint32_t j = fn[q].j;
-                    if ((int32_t)i < (int32_t)fn[q].size) {
+                    if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
                         ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
-                        const RaStep r = ra0_step(fn[q]);
-                        if (r.swd) { atomicMin(&Lmin[c][r.rdr], j); atomicMax(&Lmax[c][r.rdr], j + 1u); }
-                        if (r.sw1) { atomicMin(&Lmin[c][r.r1r], j); atomicMax(&Lmax[c][r.r1r], j + 1u); }
-                        if (r.sw2) { atomicMin(&Lmin[c][r.r2r], j); atomicMax(&Lmax[c][r.r2r], j + 1u); }
+                        ra0_lookups(Omin[c], Omax[c], ra0_step(fn[q]), j);
                     }
                 }
-                if (tid >= cb)
-                    for (uint32_t k = tid - cb; k < 128; k += nth - cb) { ot_key[c][k] = -1; ot_bits[c][k] = 0; }
                 lds_barrier();
                 if (tid < 64) {
                     uint32_t nb = 0;
                     const int32_t o = So[c][tid], v = Sv[c][tid];
-                    if (v >= 64 && o >= 0 && Lmin[c][tid] < (uint32_t)o) {
+                    if (v >= 64 && o >= 0 && min(Lmin[c][tid], Omin[c][tid]) < (uint32_t)o) {
                         if (cur_rs1[o] == (uint32_t)v) nb |= 1u;
                         if (cur_rs2[o] == (uint32_t)v) nb |= 2u;
                     }
-                    if (nb) ot_insert(ot_key[c], ot_bits[c], o, nb);
+                    flip[tid] = nb;
                     const uint64_t diff = __ballot(nb != fb);
                     fb = nb;
                     if (tid == 0) changed = diff != 0;
@@ -1236,89 +1215,93 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, cons
                 if (round > L + 1) { if (tid == 0) atomicOr(d.flags, EF_OOB); break; }
             }
         }
-        // P3: commit lifetimes; region-0 write keys (the last group wins);
-        // marks on the snapshot's virtual registers, posted to their owners
-        uint32_t rr[PER];  // committed r1r | r2r << 8 | rdr << 16
+        // P3: possible owners finalize; marks on the snapshot's virtual
+        // registers; the snapshot of step i+1 starts as a copy
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const uint32_t j = fn[q].j;
-            rr[q] = 0;
-            if ((int32_t)i < (int32_t)fn[q].size) {
+            if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
                 ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
                 const RaStep r = ra0_step(fn[q]);
-                rr[q] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
+                rrs[j] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
                 fn[q].life = r.newlife;
-                if (r.ord) atomicMax(&wkey[c][r.rdr], ((j + 1u) << 2) | 1u);
-                if (r.r1r != r.rdr) atomicMax(&wkey[c][r.r1r], ((j + 1u) << 2) | 2u);
-                if (r.r2r != r.rdr) atomicMax(&wkey[c][r.r2r], ((j + 1u) << 2) | 3u);
+                ra0_keys(wkey[c], r, j);
             }
         }
         if (tid < 64) {
             const int32_t v = Sv[c][tid], o = So[c][tid];
             Sv[n][tid] = v; So[n][tid] = o;
-            if (v >= 64 && o >= 0 && Lmax[c][tid] != 0) {
-                bool mark = true;
+            const uint32_t mx = max(Lmax[c][tid], Omax[c][tid]);
+            bool mark = false;
+            if (v >= 64 && o >= 0 && mx != 0) {
+                mark = true;
                 const uint32_t uv = (uint32_t)v;
-                if (cur_rd[o] == uv || cur_rs1[o] == uv || cur_rs2[o] == uv) mark = Lmax[c][tid] > (uint32_t)o + 1u;
-                if (mark) {
-                    uint32_t h = (uint32_t)o & 127u;
-                    for (int k = 0; k < 128; ++k) {
-                        const int32_t prev = atomicCAS(&mk_key[c][h], -1, o);
-                        if (prev == -1 || prev == o) break;
-                        h = (h + 1) & 127u;
-                    }
-                    if (tid < 32) atomicOr(&mk_lo[c][h], 1u << tid); else atomicOr(&mk_hi[c][h], 1u << (tid - 32));
-                    const uint32_t ws = (uint32_t)slot_of[o] >> 6;
-                    atomicOr(&mkw[c][ws >> 5], 1u << (ws & 31u));
-                }
+                if (cur_rd[o] == uv || cur_rs1[o] == uv || cur_rs2[o] == uv) mark = mx > (uint32_t)o + 1u;
             }
+            const uint64_t m = __ballot(mark);
+            if (tid == 0) { mk[c][0] = (uint32_t)m; mk[c][1] = (uint32_t)(m >> 32); }
         }
         lds_barrier();
-        // P4: snapshot of step i+1, commit of the symbol state, marks, and the
-        // shift to step i+1 (its operand state patched with what this step wrote)
+        // P4: snapshot of step i+1, commit of the symbol state, marks; then the
+        // fields of step i+1 into LDS and one batch of loads for the next step
+        const uint64_t marked = ((uint64_t)mk[c][1] << 32) | mk[c][0];
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const uint32_t t = tid + q * nth;
             Ra0Fn &f = fn[q];
             const uint32_t j = f.j;
             if (t >= L) continue;
-            const bool act = (int32_t)i < (int32_t)f.size;
-            if (act) {
+            uint64_t won = 0;
+            if ((int32_t)i < (int32_t)f.size) {
                 ra0_cur(f, cur_rd, cur_rs1, cur_rs2, j);
+                const uint32_t rr = rrs[j];
                 RaStep r;
                 r.ord = f.ord; r.rs1 = f.rs1; r.rs2 = f.rs2;
-                r.r1r = rr[q] & 0xFFu; r.r2r = (rr[q] >> 8) & 0xFFu; r.rdr = (rr[q] >> 16) & 0xFFu;
-                if (r.ord && wkey[c][r.rdr] == (((j + 1u) << 2) | 1u)) { Sv[n][r.rdr] = (int32_t)r.ord; So[n][r.rdr] = (int32_t)j; }
+                r.r1r = rr & 0xFFu; r.r2r = (rr >> 8) & 0xFFu; r.rdr = (rr >> 16) & 0xFFu;
+                if (r.ord && wkey[c][r.rdr] == (((j + 1u) << 2) | 1u)) {
+                    Sv[n][r.rdr] = (int32_t)r.ord; So[n][r.rdr] = (int32_t)j;
+                    if ((int32_t)r.ord >= 64) won = 1ull << r.rdr;
+                }
                 if (r.r1r != r.rdr && wkey[c][r.r1r] == (((j + 1u) << 2) | 2u)) { Sv[n][r.r1r] = -1; So[n][r.r1r] = -1; }
                 if (r.r2r != r.rdr && wkey[c][r.r2r] == (((j + 1u) << 2) | 3u)) { Sv[n][r.r2r] = -1; So[n][r.r2r] = -1; }
                 ra0_commit(d, r, (f.pk & 0xFFu) | (((f.pk >> 16) & 1u) << 8), ((f.pk >> 8) & 0xFFu) | (((f.pk >> 17) & 1u) << 8));
-                ra0_patch(f.n_rs1, f.g1, r);
-                ra0_patch(f.n_rs2, f.g2, r);
+                const bool nx = (int32_t)(i + 1) < (int32_t)f.size;
+                cur_rd[j] = nx ? f.nw.x : 0u; cur_rs1[j] = nx ? f.nw.y : 0u; cur_rs2[j] = nx ? f.nw.z : 0u;
             }
-            const uint32_t ws = t >> 6;
-            const int h = ((mkw[c][ws >> 5] >> (ws & 31u)) & 1u) ? ot_find(mk_key[c], (int32_t)j) : -1;
-            if (h >= 0) {
-                uint64_t m = ((uint64_t)mk_hi[c][h] << 32) | mk_lo[c][h];
+            if (f.own) {
+                const uint64_t o = ra0_owned(f.own, Sv[c], So[c], j);
+                uint64_t m = o & marked;
                 while (m) {
                     const uint32_t reg = (uint32_t)__builtin_ctzll(m);
                     m &= m - 1;
                     const uint32_t v = (uint32_t)Sv[c][reg];
                     if (v - 64u < d.W0) sym_mark(d, v - 64u);
                     else atomicOr(d.flags, EF_OOB);
-                    if (f.n_rs1 == v) f.g1 |= 0x100u;
-                    if (f.n_rs2 == v) f.g2 |= 0x100u;
                 }
+                f.own = o;
             }
-            if (act) {
-                f.pk = ra0_pack(f.g1, f.g2);
-                const bool nx = (int32_t)(i + 1) < (int32_t)f.size;
-                cur_rd[j] = nx ? f.n_ord : 0u; cur_rs1[j] = nx ? f.n_rs1 : 0u; cur_rs2[j] = nx ? f.n_rs2 : 0u;
-                f.n_ord = f.m_ord; f.n_rs1 = f.m_rs1; f.n_rs2 = f.m_rs2;
-                uint32_t b3;
-                ra0_fields(d, f, i + 3, b3, f.m_ord, f.m_rs1, f.m_rs2);
-                f.fp = (f.fp >> 1) | (b3 << 2);
+            f.own |= won;
+        }
+        // loads for step i+1: its operands' state (after this step's stores, in
+        // program order) and the fields of step i+2
+        uint32_t g1[PER], g2[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            Ra0Fn &f = fn[q];
+            g1[q] = g2[q] = 0;
+            if ((int32_t)(i + 1) < (int32_t)f.size) {
+                ra0_gather(d, f.nw.y, f.nw.z, g1[q], g2[q]);
+                f.fp = ra0_fpbit(f.nw.w);
+                f.nw = ra0_word(d, f, i + 2);
             }
         }
+        if (tid < 64) {
+            qmin = 0xFFFFFFFFu; qmax = 0;
+            if (i + 2 < maxsize) { qmin = gmin[(uint64_t)(i + 2) * 64u + tid]; qmax = gmax[(uint64_t)(i + 2) * 64u + tid]; }
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if ((int32_t)(i + 1) < (int32_t)fn[q].size) fn[q].pk = ra0_pack(g1[q], g2[q]);
     }
 }
 
