@@ -552,7 +552,37 @@ struct FlagScanOp {
     DEV void emit(uint64_t i, uint32_t excl, uint32_t v) const { out[i] = excl; if (v) pos[excl] = (uint32_t)i; }
 };
 
-struct SegCut { uint32_t baseL, cntL, baseR, cntR, m, cut; };
+struct SegCut { uint32_t baseL, cntL, baseR, cntR, m, cut, uni, pad; };
+
+// Ranges whose keys are all equal.  For them the introsort's moves do not
+// depend on the keys: median-of-three picks the middle element (all three
+// comparisons false: iter_swap(first, first + len/2)), the unguarded partition
+// stops at every element and swaps first+1+k with last-1-k for k < (len-1)/2,
+// and the cut falls at first+1+(len-1)/2; the final insertion sort moves none of
+// them.  So each element's final place follows from its place alone.
+// uniform_depth: partition levels until every subrange is <= 16 (the left
+// child, 1+(len-1)/2, is never the smaller one).
+DEV uint32_t uniform_depth(uint32_t len) {
+    uint32_t dpt = 0;
+    while (len > 16) { len = 1u + (len - 1u) / 2u; ++dpt; }
+    return dpt;
+}
+
+// Final place of the element at pos of an all-equal range [a, b), whose first
+// median swap has been made already.
+DEV uint32_t uniform_final(uint32_t a, uint32_t b, uint32_t pos) {
+    bool first = true;
+    while (b - a > 16) {
+        const uint32_t len = b - a;
+        if (!first) { const uint32_t mid = a + len / 2; if (pos == a) pos = mid; else if (pos == mid) pos = a; }
+        first = false;
+        const uint32_t K = (len - 1) / 2;
+        if ((pos >= a + 1 && pos < a + 1 + K) || (pos >= b - K && pos < b)) pos = a + b - pos;
+        const uint32_t cut = a + 1 + K;
+        if (pos >= cut) a = cut; else b = cut;
+    }
+    return pos;
+}
 
 DEV uint32_t scan_at(const uint32_t *sc, uint64_t i, uint64_t n, uint32_t total) { return i >= n ? total : sc[i]; }
 
@@ -576,7 +606,13 @@ __global__ void k_sort_search(const Seg *segs, uint32_t m, const uint32_t *scL, 
     if (mm + 1 <= cL) cut = Lpos[bL + mm];
     if (mm >= 1) { uint32_t r = Rpos[bR + cR - mm]; if (r < cut) cut = r; }
     if (cut <= first || cut >= last) cut = first + 1;  // unreachable after a median-of-three pivot
-    cuts[s] = SegCut{bL, cL, bR, cR, mm, cut};
+    // every element stopped on both sides: all keys equal the pivot
+    const uint32_t len = last - first;
+    if (cL == len - 1 && cR == len - 1 && uniform_depth(len) <= segs[s].limit) {
+        cuts[s] = SegCut{bL, cL, bR, cR, 0, last, 1, 0};
+        return;
+    }
+    cuts[s] = SegCut{bL, cL, bR, cR, mm, cut, 0, 0};
 }
 
 __global__ void k_sort_swap(const int32_t *seg_of, const SegCut *cuts, const uint8_t *fl, const uint32_t *scL,
@@ -593,10 +629,16 @@ __global__ void k_sort_swap(const int32_t *seg_of, const SegCut *cuts, const uin
 
 // Children of each partitioned range: next level, heapsort list, or leaves.
 __global__ void k_sort_split(const Seg *segs, const SegCut *cuts, uint32_t m, Seg *next, uint32_t *next_cnt,
-                             Seg *heap, uint32_t *heap_cnt, int32_t *child_ids) {
+                             Seg *heap, uint32_t *heap_cnt, int32_t *child_ids, Seg *uni, uint32_t *uni_cnt) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m) return;
     Seg g = segs[s];
+    if (cuts[s].uni) {  // all keys equal: placed in closed form by k_sort_rank
+        const uint32_t u = atomicAdd(uni_cnt, 1u);
+        uni[u] = g;
+        child_ids[2 * s] = child_ids[2 * s + 1] = -2 - (int32_t)u;
+        return;
+    }
     uint32_t cut = cuts[s].cut;
     uint32_t lim = g.limit - 1;
     Seg ch[2] = {{g.first, cut, lim, 0}, {cut, g.last, lim, 0}};
@@ -611,6 +653,7 @@ __global__ void k_sort_split(const Seg *segs, const SegCut *cuts, uint32_t m, Se
     }
 }
 
+// seg_of: range of the next level (>= 0), done (-1), or all-equal range u (-2 - u).
 __global__ void k_sort_segof(int32_t *seg_of, const SegCut *cuts, const int32_t *child_ids, uint64_t n) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -669,7 +712,7 @@ __global__ void k_sort_heap(const Seg *heap, const uint32_t *heap_cnt, uint32_t 
 // `bound` (1 at each range start).  Each boundary thread insertion-sorts its range.
 __global__ void k_sort_insertion(const uint8_t *bound, uint64_t n, uint32_t *si, int32_t *sk) {
     uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (b >= n || !bound[b]) return;
+    if (b >= n || bound[b] != 1) return;  // 2: an all-equal range, nothing moves
     uint64_t e = b + 1;
     while (e < n && !bound[e]) ++e;
     for (uint64_t i = b + 1; i < e; ++i) {
@@ -680,17 +723,24 @@ __global__ void k_sort_insertion(const uint8_t *bound, uint64_t n, uint32_t *si,
     }
 }
 
-// Every cut starts a new range; leaf ranges are the spans between marks.
-__global__ void k_mark_cuts(const SegCut *cuts, uint32_t m, uint8_t *bound) {
+// Every cut starts a new range; leaf ranges are the spans between marks.  An
+// all-equal range is marked 2 at its start (its end is already a mark).
+__global__ void k_mark_cuts(const Seg *segs, const SegCut *cuts, uint32_t m, uint8_t *bound) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m) return;
-    bound[cuts[s].cut] = 1;
+    if (cuts[s].uni) bound[segs[s].first] = 2;
+    else bound[cuts[s].cut] = 1;
 }
 
-__global__ void k_sort_rank(const uint32_t *si, uint64_t n, uint32_t *rank) {
+// Sorted position of every node; elements of all-equal ranges go to their
+// closed-form places.
+__global__ void k_sort_rank(const uint32_t *si, const int32_t *seg_of, const Seg *uni, uint64_t n, uint32_t *rank) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    rank[si[i]] = (uint32_t)i;
+    const int32_t s = seg_of[i];
+    uint32_t pos = (uint32_t)i;
+    if (s <= -2) { const Seg g = uni[-2 - s]; pos = uniform_final(g.first, g.last, (uint32_t)i); }
+    rank[si[i]] = pos;
 }
 
 // ================================================================= optimize

@@ -100,6 +100,23 @@ def test_gpu_single_50k(gpu_ctx):
     assert_same(r, oracle.compile_tree(t))
 
 
+@pytest.mark.parametrize("which", ["single_50k", "1M"])
+def test_gpu_sort_permutation(gpu_ctx, which):
+    # the whole depth-sort permutation (libstdc++ introsort order, including the
+    # closed-form placement of all-equal ranges) against the oracle's restatement
+    t = W.single_50k() if which == "single_50k" else W.program_of_size(0x5EED0007, 1_000_000)
+    say("sort permutation", which, t.n)
+    r = gpu_ctx.compile(t)
+    assert r["tie_slots"] > 0  # the permutation is only computed when equal-depth writers collide
+    o = oracle.Oracle(t)
+    o.run(3)
+    want = o.get("sorted")
+    got = gpu_ctx.dump("sorted")
+    assert got.shape == want.shape
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"{bad.size} positions differ, first at {bad[0]}"
+
+
 def test_gpu_one_million_nodes(gpu_ctx):
     t = W.program_of_size(0x5EED0007, 1_000_000)
     say("1M-node program", t.n)
