@@ -503,14 +503,16 @@ __global__ __launch_bounds__(256) void k_gen_write(Dev d, int use_rank) {
 // final insertion sort, heapsort where the limit runs out.  The k-th left stop
 // (key >= pivot, scanning up from first+1) swaps with the k-th right stop
 // (key <= pivot, scanning down from last-1) while they have not crossed, so one
-// partition is two stream compactions and a search; all ranges of one
-// recursion depth are partitioned together.
-struct Seg { uint32_t first, last, limit, pad; };
+// partition is a stream compaction of the stops and a search; all ranges of one
+// recursion depth are partitioned together.  Each level works on the list of
+// its ranges only (their positions concatenated: the level's active list,
+// offsets in Seg::off, ranges in position order); ranges that are done leave it.
+struct Seg { uint32_t first, last, limit, off; };
 
-__global__ void k_sort_init(uint32_t *sidx, int32_t *skey, const int32_t *depth, uint64_t n, int32_t *seg_of) {
+__global__ void k_sort_init(uint32_t *sidx, int32_t *skey, const int32_t *depth, uint64_t n) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    sidx[i] = (uint32_t)i; skey[i] = depth[i]; seg_of[i] = 0;
+    sidx[i] = (uint32_t)i; skey[i] = depth[i];
 }
 
 DEV void swap_pos(uint32_t *sidx, int32_t *skey, uint32_t a, uint32_t b) {
@@ -518,10 +520,8 @@ DEV void swap_pos(uint32_t *sidx, int32_t *skey, uint32_t a, uint32_t b) {
     int32_t tk = skey[a]; skey[a] = skey[b]; skey[b] = tk;
 }
 
-__global__ void k_sort_median(Seg *segs, uint32_t m, uint32_t *sidx, int32_t *skey, int32_t *pivot) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m) return;
-    uint32_t first = segs[s].first, last = segs[s].last;
+// std::__move_median_to_first(first, first+1, mid, last-1); returns the pivot key.
+DEV int32_t median_to_first(uint32_t first, uint32_t last, uint32_t *sidx, int32_t *skey) {
     uint32_t a = first + 1, b = first + (last - first) / 2, c = last - 1;
     int32_t ka = skey[a], kb = skey[b], kc = skey[c];
     uint32_t pick;
@@ -530,27 +530,141 @@ __global__ void k_sort_median(Seg *segs, uint32_t m, uint32_t *sidx, int32_t *sk
     else if (kb < kc) pick = c;
     else pick = b;
     swap_pos(sidx, skey, first, pick);
-    pivot[s] = skey[first];
+    return skey[first];
 }
 
-__global__ void k_sort_flags(const int32_t *seg_of, const Seg *segs, const int32_t *pivot, const int32_t *skey,
-                             uint64_t n, uint8_t *fl, uint8_t *fr) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t s = seg_of[i];
-    uint8_t l = 0, r = 0;
-    if (s >= 0 && i > segs[s].first) {
-        int32_t p = pivot[s], k = skey[i];
-        l = !(k < p); r = !(p < k);
+__global__ void k_sort_root(const Seg *segs, uint32_t *sidx, int32_t *skey, int32_t *pivot) {
+    pivot[0] = median_to_first(segs[0].first, segs[0].last, sidx, skey);
+}
+
+// Last range whose active offset is <= a.
+DEV uint32_t seg_find(const Seg *segs, uint32_t m, uint32_t a) {
+    uint32_t lo = 0, hi = m;
+    while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (segs[mid].off <= a) lo = mid; else hi = mid; }
+    return lo;
+}
+
+// Walks SCAN_ITEMS consecutive active indices from a0, tracking their range.
+struct SegWalk {
+    const Seg *segs; uint32_t m, s, nxt; Seg g;
+    DEV SegWalk(const Seg *sg, uint32_t mm, uint32_t a0) : segs(sg), m(mm) {
+        s = seg_find(sg, mm, a0); g = sg[s]; nxt = s + 1 < mm ? sg[s + 1].off : 0xFFFFFFFFu;
     }
-    fl[i] = l; fr[i] = r;
+    DEV void to(uint32_t a) {
+        while (a >= nxt) { ++s; g = segs[s]; nxt = s + 1 < m ? segs[s + 1].off : 0xFFFFFFFFu; }
+    }
+    DEV uint32_t pos(uint32_t a) const { return g.first + (a - g.off); }
+};
+
+// Left/right stop flags of the element at pos of range g (pivot key p): the
+// pivot itself, at first, takes part in neither.
+DEV uint32_t stop_bits(const Seg &g, int32_t p, const int32_t *skey, uint32_t i) {
+    if (i == g.first) return 0;
+    const int32_t k = skey[i];
+    return (!(k < p) ? 1u : 0u) | (!(p < k) ? 2u : 0u);
 }
 
-struct FlagScanOp {
-    const uint8_t *f; uint32_t *out; uint32_t *pos;
-    DEV uint32_t value(uint64_t i) const { return f[i]; }
-    DEV void emit(uint64_t i, uint32_t excl, uint32_t v) const { out[i] = excl; if (v) pos[excl] = (uint32_t)i; }
-};
+// Per-tile counts of left and right stops over the active list.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_reduce(const Seg *segs, uint32_t m, uint32_t A,
+                                                               const int32_t *pivot, const int32_t *skey,
+                                                               uint2 *tile_sums) {
+    __shared__ uint32_t sL[SCAN_BLOCK / 64], sR[SCAN_BLOCK / 64];
+    const uint32_t a0 = blockIdx.x * (uint32_t)SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t cl = 0, cr = 0;
+    if (a0 < A) {
+        SegWalk w(segs, m, a0);
+        int32_t p = pivot[w.s];
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            const uint32_t a = a0 + k;
+            if (a >= A) break;
+            const uint32_t s0 = w.s;
+            w.to(a);
+            if (w.s != s0) p = pivot[w.s];
+            const uint32_t b = stop_bits(w.g, p, skey, w.pos(a));
+            cl += b & 1u; cr += b >> 1;
+        }
+    }
+    cl = wave_sum(cl); cr = wave_sum(cr);
+    if ((threadIdx.x & 63) == 0) { sL[threadIdx.x >> 6] = cl; sR[threadIdx.x >> 6] = cr; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tl = 0, tr = 0;
+#pragma unroll
+        for (int w = 0; w < SCAN_BLOCK / 64; ++w) { tl += sL[w]; tr += sR[w]; }
+        tile_sums[blockIdx.x] = make_uint2(tl, tr);
+    }
+}
+
+// One workgroup: exclusive scan of (left, right) tile sums in place; totals to *total.
+__global__ __launch_bounds__(1024) void k_scan_tiles2(uint2 *sums, uint32_t ntiles, uint2 *total) {
+    __shared__ uint32_t wl[16], wr[16];
+    __shared__ uint32_t cl, cr;
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    if (threadIdx.x == 0) { cl = 0; cr = 0; }
+    __syncthreads();
+    for (uint32_t b = 0; b < ntiles; b += 1024) {
+        const uint32_t i = b + threadIdx.x;
+        const uint2 v = i < ntiles ? sums[i] : make_uint2(0, 0);
+        const uint32_t xl = wave_incl_scan(v.x, lane), xr = wave_incl_scan(v.y, lane);
+        if (lane == 63) { wl[wid] = xl; wr[wid] = xr; }
+        __syncthreads();
+        uint32_t pl = cl, pr = cr, al = 0, ar = 0;
+        for (uint32_t w = 0; w < 16; ++w) {
+            if (w < wid) { pl += wl[w]; pr += wr[w]; }
+            al += wl[w]; ar += wr[w];
+        }
+        if (i < ntiles) sums[i] = make_uint2(pl + xl - v.x, pr + xr - v.y);
+        __syncthreads();
+        if (threadIdx.x == 0) { cl += al; cr += ar; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = make_uint2(cl, cr);
+}
+
+// Compaction of the stops: Lpos[k] / Rpos[k] = position of the k-th left / right
+// stop of the level; base[s] = counts before range s.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_down(const Seg *segs, uint32_t m, uint32_t A,
+                                                             const int32_t *pivot, const int32_t *skey,
+                                                             const uint2 *tile_pre, uint32_t *Lpos, uint32_t *Rpos,
+                                                             uint2 *base) {
+    __shared__ uint32_t sL[SCAN_BLOCK / 64], sR[SCAN_BLOCK / 64];
+    const uint32_t a0 = blockIdx.x * (uint32_t)SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t bits = 0, cl = 0, cr = 0;  // 2 bits per item
+    if (a0 < A) {
+        SegWalk w(segs, m, a0);
+        int32_t p = pivot[w.s];
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            const uint32_t a = a0 + k;
+            if (a >= A) break;
+            const uint32_t s0 = w.s;
+            w.to(a);
+            if (w.s != s0) p = pivot[w.s];
+            const uint32_t b = stop_bits(w.g, p, skey, w.pos(a));
+            bits |= b << (2 * k);
+            cl += b & 1u; cr += b >> 1;
+        }
+    }
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    const uint32_t xl = wave_incl_scan(cl, lane), xr = wave_incl_scan(cr, lane);
+    if (lane == 63) { sL[wid] = xl; sR[wid] = xr; }
+    __syncthreads();
+    uint32_t rl = tile_pre[blockIdx.x].x + xl - cl, rr = tile_pre[blockIdx.x].y + xr - cr;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) if ((uint32_t)w < wid) { rl += sL[w]; rr += sR[w]; }
+    if (a0 < A) {
+        SegWalk w(segs, m, a0);
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            const uint32_t a = a0 + k;
+            if (a >= A) break;
+            w.to(a);
+            const uint32_t i = w.pos(a);
+            if (a == w.g.off) base[w.s] = make_uint2(rl, rr);
+            const uint32_t b = (bits >> (2 * k)) & 3u;
+            if (b & 1u) Lpos[rl++] = i;
+            if (b & 2u) Rpos[rr++] = i;
+        }
+    }
+}
 
 struct SegCut { uint32_t baseL, cntL, baseR, cntR, m, cut, uni, pad; };
 
@@ -584,17 +698,13 @@ DEV uint32_t uniform_final(uint32_t a, uint32_t b, uint32_t pos) {
     return pos;
 }
 
-DEV uint32_t scan_at(const uint32_t *sc, uint64_t i, uint64_t n, uint32_t total) { return i >= n ? total : sc[i]; }
-
-__global__ void k_sort_search(const Seg *segs, uint32_t m, const uint32_t *scL, const uint32_t *scR,
-                              const uint32_t *Lpos, const uint32_t *Rpos, uint64_t n, const uint32_t *totals,
-                              SegCut *cuts) {
+__global__ void k_sort_search(const Seg *segs, uint32_t m, const uint2 *base, const uint2 *total,
+                              const uint32_t *Lpos, const uint32_t *Rpos, SegCut *cuts) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m) return;
     uint32_t first = segs[s].first, last = segs[s].last;
-    uint32_t bL = scan_at(scL, first, n, totals[0]), eL = scan_at(scL, last, n, totals[0]);
-    uint32_t bR = scan_at(scR, first, n, totals[1]), eR = scan_at(scR, last, n, totals[1]);
-    uint32_t cL = eL - bL, cR = eR - bR;
+    const uint2 b0 = base[s], b1 = s + 1 < m ? base[s + 1] : *total;
+    uint32_t bL = b0.x, bR = b0.y, cL = b1.x - b0.x, cR = b1.y - b0.y;
     // m = largest k with L_k < R_k (1-based), L_k = Lpos[bL+k-1], R_k = Rpos[bR+cR-k]
     uint32_t lo = 0, hi = cL < cR ? cL : cR;
     while (lo < hi) {
@@ -615,51 +725,88 @@ __global__ void k_sort_search(const Seg *segs, uint32_t m, const uint32_t *scL, 
     cuts[s] = SegCut{bL, cL, bR, cR, mm, cut, 0, 0};
 }
 
-__global__ void k_sort_swap(const int32_t *seg_of, const SegCut *cuts, const uint8_t *fl, const uint32_t *scL,
-                            const uint32_t *Rpos, uint64_t n, uint32_t *sidx, int32_t *skey) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= n || !fl[i]) return;
-    int32_t s = seg_of[i];
-    SegCut c = cuts[s];
-    uint32_t k = scL[i] - c.baseL + 1;
-    if (k > c.m) return;
-    uint32_t partner = Rpos[c.baseR + c.cntR - k];
-    swap_pos(sidx, skey, (uint32_t)i, partner);
+// The partition's swaps: pair k of a range (1-based) swaps its k-th left and
+// k-th-from-the-end right stop; the index within the range is the pair number.
+// A workgroup covers 256*SCAN_ITEMS active indices, consecutive lanes taking
+// consecutive pairs (coalesced stop lists).
+__global__ __launch_bounds__(256) void k_sort_swap(const Seg *segs, uint32_t m, uint32_t A, const SegCut *cuts,
+                                                   const uint32_t *Lpos, const uint32_t *Rpos, uint32_t *sidx,
+                                                   int32_t *skey) {
+    const uint32_t a0 = blockIdx.x * 256u * SCAN_ITEMS + threadIdx.x;
+    if (a0 >= A) return;
+    SegWalk w(segs, m, a0);
+    SegCut c = cuts[w.s];
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const uint32_t a = a0 + 256u * k;
+        if (a >= A) break;
+        const uint32_t s0 = w.s;
+        w.to(a);
+        if (w.s != s0) c = cuts[w.s];
+        const uint32_t kk = a - w.g.off + 1;
+        if (kk <= c.m) swap_pos(sidx, skey, Lpos[c.baseL + kk - 1], Rpos[c.baseR + c.cntR - kk]);
+    }
 }
 
-// Children of each partitioned range: next level, heapsort list, or leaves.
-__global__ void k_sort_split(const Seg *segs, const SegCut *cuts, uint32_t m, Seg *next, uint32_t *next_cnt,
-                             Seg *heap, uint32_t *heap_cnt, int32_t *child_ids, Seg *uni, uint32_t *uni_cnt) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m) return;
-    Seg g = segs[s];
-    if (cuts[s].uni) {  // all keys equal: placed in closed form by k_sort_rank
-        const uint32_t u = atomicAdd(uni_cnt, 1u);
-        uni[u] = g;
-        child_ids[2 * s] = child_ids[2 * s + 1] = -2 - (int32_t)u;
-        return;
-    }
-    uint32_t cut = cuts[s].cut;
-    uint32_t lim = g.limit - 1;
-    Seg ch[2] = {{g.first, cut, lim, 0}, {cut, g.last, lim, 0}};
-    for (int c = 0; c < 2; ++c) {
-        uint32_t sz = ch[c].last - ch[c].first;
-        int32_t id = -1;
-        if (sz > 16) {
-            if (lim == 0) { uint32_t h = atomicAdd(heap_cnt, 1u); heap[h] = ch[c]; }
-            else { uint32_t q = atomicAdd(next_cnt, 1u); next[q] = ch[c]; id = (int32_t)q; }
+struct SortCnt { uint32_t m, A, nheap, nuni, uni_total, pad[3]; };
+
+DEV uint32_t block_excl_scan(uint32_t v, uint32_t *tmp /* 16 */, uint32_t &total) {
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    const uint32_t x = wave_incl_scan(v, lane);
+    if (lane == 63) tmp[wid] = x;
+    __syncthreads();
+    uint32_t pre = 0, all = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) { if (w < wid) pre += tmp[w]; all += tmp[w]; }
+    __syncthreads();
+    total = all;
+    return pre + x - v;
+}
+
+// One workgroup: the children of every partitioned range, in position order:
+// next level (with their active offsets), heapsort list, or leaves (marked in
+// `bound`); all-equal ranges to the closed-form list (offsets in its own
+// element order); then median-of-three for every range of the next level.
+__global__ __launch_bounds__(1024) void k_sort_plan(const Seg *segs, const SegCut *cuts, uint32_t m, Seg *next,
+                                                    int32_t *next_pivot, Seg *heap, Seg *uni, SortCnt *cnt,
+                                                    uint8_t *bound, uint32_t *sidx, int32_t *skey) {
+    __shared__ uint32_t tmp[16];
+    uint32_t q0 = 0, a0 = 0, h0 = cnt->nheap, u0 = cnt->nuni, e0 = cnt->uni_total;
+    for (uint32_t b = 0; b < m; b += blockDim.x) {
+        const uint32_t s = b + threadIdx.x;
+        Seg g{0, 0, 0, 0};
+        SegCut ct{};
+        if (s < m) { g = segs[s]; ct = cuts[s]; }
+        const bool v = s < m, un = v && ct.uni;
+        const uint32_t lim = g.limit - 1;
+        const uint32_t c0 = ct.cut - g.first, c1 = g.last - ct.cut;
+        const bool n0 = v && !un && c0 > 16 && lim > 0, n1 = v && !un && c1 > 16 && lim > 0;
+        const bool h0b = v && !un && c0 > 16 && lim == 0, h1b = v && !un && c1 > 16 && lim == 0;
+        uint32_t tq, ta, th, tu, te;
+        const uint32_t xq = block_excl_scan((n0 ? 1u : 0u) + (n1 ? 1u : 0u), tmp, tq);
+        const uint32_t xa = block_excl_scan((n0 ? c0 : 0u) + (n1 ? c1 : 0u), tmp, ta);
+        const uint32_t xh = block_excl_scan((h0b ? 1u : 0u) + (h1b ? 1u : 0u), tmp, th);
+        const uint32_t xu = block_excl_scan(un ? 1u : 0u, tmp, tu);
+        const uint32_t xe = block_excl_scan(un ? g.last - g.first : 0u, tmp, te);
+        if (v) {
+            if (un) {
+                uni[u0 + xu] = Seg{g.first, g.last, g.limit, e0 + xe};
+                bound[g.first] = 2;
+            } else {
+                bound[ct.cut] = 1;
+                uint32_t q = q0 + xq, a = a0 + xa, h = h0 + xh;
+                if (n0) { next[q++] = Seg{g.first, ct.cut, lim, a}; a += c0; }
+                if (n1) { next[q++] = Seg{ct.cut, g.last, lim, a}; }
+                if (h0b) heap[h++] = Seg{g.first, ct.cut, 0, 0};
+                if (h1b) heap[h++] = Seg{ct.cut, g.last, 0, 0};
+            }
         }
-        child_ids[2 * s + c] = id;
+        q0 += tq; a0 += ta; h0 += th; u0 += tu; e0 += te;
     }
-}
-
-// seg_of: range of the next level (>= 0), done (-1), or all-equal range u (-2 - u).
-__global__ void k_sort_segof(int32_t *seg_of, const SegCut *cuts, const int32_t *child_ids, uint64_t n) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t s = seg_of[i];
-    if (s < 0) return;
-    seg_of[i] = child_ids[2 * s + (i >= cuts[s].cut ? 1 : 0)];
+    __syncthreads();
+    if (threadIdx.x == 0) { cnt->m = q0; cnt->A = a0; cnt->nheap = h0; cnt->nuni = u0; cnt->uni_total = e0; }
+    __threadfence_block();
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < q0; q += blockDim.x)
+        next_pivot[q] = median_to_first(next[q].first, next[q].last, sidx, skey);
 }
 
 DEV bool KLESS(const int32_t *skey, uint32_t a, uint32_t b) { return skey[a] < skey[b]; }
@@ -723,24 +870,21 @@ __global__ void k_sort_insertion(const uint8_t *bound, uint64_t n, uint32_t *si,
     }
 }
 
-// Every cut starts a new range; leaf ranges are the spans between marks.  An
-// all-equal range is marked 2 at its start (its end is already a mark).
-__global__ void k_mark_cuts(const Seg *segs, const SegCut *cuts, uint32_t m, uint8_t *bound) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m) return;
-    if (cuts[s].uni) bound[segs[s].first] = 2;
-    else bound[cuts[s].cut] = 1;
-}
-
-// Sorted position of every node; elements of all-equal ranges go to their
-// closed-form places.
-__global__ void k_sort_rank(const uint32_t *si, const int32_t *seg_of, const Seg *uni, uint64_t n, uint32_t *rank) {
+__global__ void k_sort_rank(const uint32_t *si, uint64_t n, uint32_t *rank) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int32_t s = seg_of[i];
-    uint32_t pos = (uint32_t)i;
-    if (s <= -2) { const Seg g = uni[-2 - s]; pos = uniform_final(g.first, g.last, (uint32_t)i); }
-    rank[si[i]] = pos;
+    rank[si[i]] = (uint32_t)i;
+}
+
+// Elements of all-equal ranges go to their closed-form places (after the
+// plain ranks, which they overwrite).
+__global__ void k_sort_rank_uniform(const Seg *uni, uint32_t nuni, uint32_t total, const uint32_t *si,
+                                    uint32_t *rank) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const Seg g = uni[seg_find(uni, nuni, e)];
+    const uint32_t p = g.first + (e - g.off);
+    rank[si[p]] = uniform_final(g.first, g.last, p);
 }
 
 // ================================================================= optimize
@@ -755,19 +899,35 @@ __global__ __launch_bounds__(256) void k_opt_init(Dev d) {
 }
 
 // One sweep: a word whose destination register is unused clears its sources
-// (except sources of stores).  Repeated until a sweep changes nothing.
+// (except sources of stores).  Repeated until a sweep changes nothing.  A
+// word's sources are mostly the words just before it, so each workgroup
+// repeats its own range of OPT_RANGE words until no word of the range lost its
+// use; only chains crossing ranges need further sweeps.
+constexpr uint32_t OPT_RANGE = 256 * 8;
+
 __global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= d.W0) return;
-    uint32_t rd = d.w0_rd[i];
-    if (!((int32_t)rd > 63)) return;
-    uint32_t v = rd - 64u;
-    if (v >= d.W0 || d.used[v]) return;
-    int32_t a = (int32_t)(d.w0_rs1[i] - 64u), b = (int32_t)(d.w0_rs2[i] - 64u);
-    bool ch = false;
-    if (a >= 0 && (uint32_t)a < d.W0 && !d.sstore[a] && d.used[a]) { d.used[a] = 0; ch = true; }
-    if (b >= 0 && (uint32_t)b < d.W0 && !d.sstore[b] && d.used[b]) { d.used[b] = 0; ch = true; }
-    if (ch) *changed = 1;
+    __shared__ int local;
+    const uint64_t b0 = (uint64_t)blockIdx.x * OPT_RANGE;
+    const uint64_t b1 = b0 + OPT_RANGE < d.W0 ? b0 + OPT_RANGE : d.W0;
+    bool any = false;
+    for (int round = 0; round < 64; ++round) {
+        if (threadIdx.x == 0) local = 0;
+        __syncthreads();
+        bool loc = false;
+        for (uint64_t i = b1 - 1 - threadIdx.x; (int64_t)i >= (int64_t)b0; i -= 256) {
+            const uint32_t rd = d.w0_rd[i];
+            if (!((int32_t)rd > 63)) continue;
+            const uint32_t v = rd - 64u;
+            if (v >= d.W0 || d.used[v]) continue;
+            const int32_t a = (int32_t)(d.w0_rs1[i] - 64u), b = (int32_t)(d.w0_rs2[i] - 64u);
+            if (a >= 0 && (uint32_t)a < d.W0 && !d.sstore[a] && d.used[a]) { d.used[a] = 0; any = true; loc |= (uint64_t)a >= b0 && (uint64_t)a < b1; }
+            if (b >= 0 && (uint32_t)b < d.W0 && !d.sstore[b] && d.used[b]) { d.used[b] = 0; any = true; loc |= (uint64_t)b >= b0 && (uint64_t)b < b1; }
+        }
+        if (loc) local = 1;
+        __syncthreads();
+        if (!local) break;
+    }
+    if (any) *changed = 1;
 }
 
 __global__ __launch_bounds__(256) void k_opt_final(Dev d) {
@@ -869,10 +1029,65 @@ DEV void ra_commit_syms(const Dev &d, const RaStep &r) {
 // (only lane-0 registers are ever looked up), so each walks alone.  Per step,
 // the registers each group looks up are recorded as min/max group per register.
 // permw lists these functions by size, largest first, so that the 64 walks of
-// a workgroup take about the same number of steps.
-__global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin, uint32_t *gmax, const uint32_t *permw,
-                                                uint32_t nw) {
+// a workgroup take about the same number of steps.  A function's virtual
+// registers are named by that function alone (k_xref_check), so each walk keeps
+// them in a write-back LDS cache of WALK_CACHE entries (slot = word offset in
+// the function mod WALK_CACHE; entry = offset << 10 | valid << 9 | state) and
+// touches HBM only on a miss or an eviction.  The words themselves are staged
+// into LDS WALK_CHUNK steps at a time, loaded cooperatively (consecutive
+// threads, consecutive words of one function).
+constexpr uint32_t WALK_CACHE = 64;
+constexpr uint32_t WALK_CHUNK = 32;
+// Workgroups spread their per-step lookup records over WALK_COPIES copies of
+// the (step, register) table (copy = workgroup mod WALK_COPIES), which
+// k_walk_merge folds: one table shared by every workgroup serializes its
+// atomics on the same few addresses.
+constexpr uint32_t WALK_COPIES = 16;
+
+// The walk's cache of a function's states, kept by one thread: ring slot x & 7
+// holds the newest definitions (tag x << 10 | valid << 9 | state); the LDS
+// cache c[x % WALK_CACHE][lane] holds older ones; HBM (sym) the rest.
+// The ring is eight named registers (an array would be indexed through
+// scratch memory).
+struct WcRing { uint32_t r0, r1, r2, r3, r4, r5, r6, r7; };
+
+DEV uint32_t wc_ring_at(const WcRing &g, uint32_t k) {
+    const uint32_t a = (k & 1u) ? g.r1 : g.r0, b = (k & 1u) ? g.r3 : g.r2;
+    const uint32_t c = (k & 1u) ? g.r5 : g.r4, e = (k & 1u) ? g.r7 : g.r6;
+    const uint32_t lo = (k & 2u) ? b : a, hi = (k & 2u) ? e : c;
+    return (k & 4u) ? hi : lo;
+}
+
+DEV void wc_ring_set(WcRing &g, uint32_t k, uint32_t v) {
+    g.r0 = k == 0 ? v : g.r0; g.r1 = k == 1 ? v : g.r1; g.r2 = k == 2 ? v : g.r2; g.r3 = k == 3 ? v : g.r3;
+    g.r4 = k == 4 ? v : g.r4; g.r5 = k == 5 ? v : g.r5; g.r6 = k == 6 ? v : g.r6; g.r7 = k == 7 ? v : g.r7;
+}
+
+DEV void wc_evict(const Dev &d, uint32_t start, uint32_t e) {
+    if (e) d.sym[start + (e >> 10)] = (uint16_t)(e & 0x1FFu);
+}
+
+DEV uint32_t wc_lds_get(const Dev &d, uint32_t (*c)[64], uint32_t lane, uint32_t start, uint32_t x) {
+    const uint32_t e = c[x % WALK_CACHE][lane];
+    if (e && (e >> 10) == x) return e & 0x1FFu;
+    wc_evict(d, start, e);
+    const uint32_t val = d.sym[start + x];
+    c[x % WALK_CACHE][lane] = (x << 10) | 0x200u | val;
+    return val;
+}
+
+DEV void wc_lds_put(const Dev &d, uint32_t (*c)[64], uint32_t lane, uint32_t start, uint32_t x, uint32_t val) {
+    const uint32_t e = c[x % WALK_CACHE][lane];
+    if (e && (e >> 10) != x) wc_evict(d, start, e);
+    c[x % WALK_CACHE][lane] = (x << 10) | 0x200u | val;
+}
+
+__global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint32_t *gmax_all, uint64_t stride,
+                                                const uint32_t *permw, uint32_t nw) {
     __shared__ uint32_t smin[64], smax[64];
+    __shared__ uint32_t cache[WALK_CACHE][64];
+    __shared__ uint4 words[WALK_CHUNK][64];
+    __shared__ uint32_t fstart[64], fsize[64];
     const uint32_t lane = threadIdx.x;
     const uint32_t t = blockIdx.x * 64u + lane;
     const uint32_t f = t < nw ? permw[t] : d.F;
@@ -880,21 +1095,71 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin, uint32_t 
     const uint32_t start = act ? d.fstarts[f] : 0u;
     const uint32_t size = act ? d.fsizes[f] : 0u;
     const uint32_t grp = f >> 3;
+    uint32_t *gmin = gmin_all + (blockIdx.x % WALK_COPIES) * stride;
+    uint32_t *gmax = gmax_all + (blockIdx.x % WALK_COPIES) * stride;
+    const bool con = size < (1u << 22);  // cache on (tags hold 22 bits of word offset)
+    WcRing ring{0, 0, 0, 0, 0, 0, 0, 0};
+    // state of virtual register v (inside [64, 64 + W0)); then its update
+    auto get = [&](uint32_t v) -> uint32_t {
+        const uint32_t x = v - 64u - start;
+        if (!con || x >= size) return d.sym[v - 64u];
+        const uint32_t e = wc_ring_at(ring, x & 7u);
+        if (e && (e >> 10) == x) return e & 0x1FFu;
+        return wc_lds_get(d, cache, lane, start, x);
+    };
+    auto put = [&](uint32_t v, uint32_t val, bool own) {  // own: v is defined by the word being committed
+        const uint32_t x = v - 64u - start;
+        if (!con || x >= size) { d.sym[v - 64u] = (uint16_t)val; return; }
+        const uint32_t e = wc_ring_at(ring, x & 7u);
+        if (e && (e >> 10) == x) { wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val); return; }
+        if (!own) { wc_lds_put(d, cache, lane, start, x, val); return; }
+        if (e) wc_lds_put(d, cache, lane, start, e >> 10, e & 0x1FFu);
+        wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val);
+    };
+    for (uint32_t k = 0; k < WALK_CACHE; ++k) cache[k][lane] = 0;
+    fstart[lane] = start; fsize[lane] = size;
     smin[lane] = 0xFFFFFFFFu; smax[lane] = 0;
     uint32_t wmax = size;
     for (int s = 32; s > 0; s >>= 1) { uint32_t o = __shfl_xor(wmax, s, 64); wmax = wmax > o ? wmax : o; }
     uint64_t life = 0x1Full;
     __syncthreads();
     for (uint32_t i = 0; i < wmax; ++i) {
+        if (i % WALK_CHUNK == 0) {
+            // thread t stages word i + t % 32 of functions t / 32, t / 32 + 2, ...: two batches of 16 loads
+            const uint32_t k = lane % WALK_CHUNK, g0 = lane / WALK_CHUNK;
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                uint4 tmp[16];
+#pragma unroll
+                for (uint32_t r = 0; r < 16; ++r) {
+                    const uint32_t g = g0 + 2u * (16u * h + r);
+                    tmp[r] = i + k < fsize[g] ? d.w0_f4[fstart[g] + i + k] : make_uint4(0, 0, 0, 0);
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < 16; ++r) words[k][g0 + 2u * (16u * h + r)] = tmp[r];
+            }
+            __syncthreads();
+        }
         if ((int32_t)i < (int32_t)size) {
-            const uint32_t off = start + i;
-            RaStep r = ra_eval(d, off, life, false, false);
+            const uint4 w = words[i % WALK_CHUNK][lane];
+            const uint32_t raw1 = virt_ok(d, w.y) ? (get(w.y) & 0xFFu) : 0u;
+            const uint32_t raw2 = virt_ok(d, w.z) ? (get(w.z) & 0xFFu) : 0u;
+            const RaStep r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1, raw2, false, false, life);
             xref_check_fields(d, f, start, size, r.ord, r.rs1, r.rs2);
             if (r.swd) { atomicMin(&smin[r.rdr], grp); atomicMax(&smax[r.rdr], grp + 1u); }
             if (r.sw1) { atomicMin(&smin[r.r1r], grp); atomicMax(&smax[r.r1r], grp + 1u); }
             if (r.sw2) { atomicMin(&smin[r.r2r], grp); atomicMax(&smax[r.r2r], grp + 1u); }
             life = r.newlife;
-            ra_commit_syms(d, r);
+            // register_state updates (ra_commit_syms): destination, then rs1, then rs2
+            if (((int32_t)r.ord > 63 && r.ord - 64u >= d.W0) || ((int32_t)r.rs1 > 63 && r.rs1 - 64u >= d.W0) ||
+                ((int32_t)r.rs2 > 63 && r.rs2 - 64u >= d.W0)) {
+                atomicOr(d.flags, EF_OOB);
+            } else {
+                const uint32_t self = 64u + start + i;
+                if ((int32_t)r.ord > 63) put(r.ord, r.rdr, r.ord == self);
+                if ((int32_t)r.rs1 > 63) put(r.rs1, r.r1r, false);
+                if ((int32_t)r.rs2 > 63) put(r.rs2, r.r2r, false);
+            }
         }
         lds_barrier();
         uint32_t mx = smax[lane];
@@ -905,6 +1170,25 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin, uint32_t 
         }
         lds_barrier();
     }
+    if (con) {
+        for (uint32_t q = 0; q < 8; ++q) {
+            const uint32_t e = wc_ring_at(ring, q);
+            if (e) wc_lds_put(d, cache, lane, start, e >> 10, e & 0x1FFu);
+        }
+        for (uint32_t q = 0; q < WALK_CACHE; ++q) wc_evict(d, start, cache[q][lane]);
+    }
+}
+
+// gmin/gmax[e] = min/max over the WALK_COPIES copies (copy c at c * stride).
+__global__ void k_walk_merge(uint32_t *gmin, uint32_t *gmax, uint64_t stride) {
+    const uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (e >= stride) return;
+    uint32_t mn = gmin[e], mx = gmax[e];
+    for (uint32_t c = 1; c < WALK_COPIES; ++c) {
+        mn = min(mn, gmin[c * stride + e]);
+        mx = max(mx, gmax[c * stride + e]);
+    }
+    gmin[e] = mn; gmax[e] = mx;
 }
 
 // Lane-0 functions in step lockstep, one workgroup.  Region 0 of the
@@ -1352,6 +1636,58 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, cons
 #pragma unroll
         for (int q = 0; q < PER; ++q)
             if ((int32_t)(i + 1) < (int32_t)fn[q].size) fn[q].pk = ra0_pack(g1[q], g2[q]);
+    }
+}
+
+// Exact fallback for programs in which two functions name the same virtual
+// register (see xref_note): the reference's allocation walk as written
+// (rv_generator_avx.cpp:1041-1418 at one thread), one wave: steps in order,
+// groups of 8 functions in order within a step, the 8 functions of a group on
+// lanes 0-7.  Each group body reads its operands' states before any of its
+// writes; its writes follow the reference's order (marks, lifetimes,
+// register_state, then the symbol entries of rd, rs1, rs2, lane by lane).
+// register_state keeps 8 regions of 64 (region = lane); lookups read region 0
+// of the step's snapshot.
+__global__ __launch_bounds__(64) void k_ra_serial(Dev d, uint32_t maxsize, uint64_t *life) {
+    __shared__ int32_t reg[8][64];
+    __shared__ int32_t snap[64];
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t k = lane; k < 8 * 64; k += 64) reg[k / 64][k % 64] = -1;
+    for (uint32_t f = lane; f < d.F; f += 64) life[f] = 0x1Full;
+    __syncthreads();
+    const uint32_t G = (d.F + 7u) / 8u;
+    for (uint32_t i = 0; i < maxsize; ++i) {
+        snap[lane] = reg[0][lane];
+        __syncthreads();
+        for (uint32_t g = 0; g < G; ++g) {
+            const uint32_t f = 8u * g + lane;
+            const bool nc = lane < 8 && f < d.F && (int32_t)i < (int32_t)d.fsizes[f];
+            uint4 w = make_uint4(0, 0, 0, 0);
+            uint64_t lt = 0;
+            if (nc) { w = d.w0_f4[d.fstarts[f] + i]; lt = life[f]; }
+            const uint32_t raw1 = nc && virt_ok(d, w.y) ? d.sym[w.y - 64u] : 0u;
+            const uint32_t raw2 = nc && virt_ok(d, w.z) ? d.sym[w.z - 64u] : 0u;
+            const RaStep r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1 & 0xFFu, raw2 & 0xFFu, (raw1 >> 8) != 0,
+                                         (raw2 >> 8) != 0, lt);
+            __syncthreads();  // every lane has read before any lane writes
+            if (nc && r.swd) { const int32_t x = snap[r.rdr] - 64; if (x >= 0) sym_mark(d, (uint32_t)x); }
+            if (nc && r.sw1) { const int32_t x = snap[r.r1r] - 64; if (x >= 0) sym_mark(d, (uint32_t)x); }
+            if (nc && r.sw2) { const int32_t x = snap[r.r2r] - 64; if (x >= 0) sym_mark(d, (uint32_t)x); }
+            if (nc) {
+                life[f] = r.newlife;
+                if (r.ord) reg[lane][r.rdr] = (int32_t)r.ord;
+                if (r.r1r != r.rdr) reg[lane][r.r1r] = -1;
+                if (r.r2r != r.rdr) reg[lane][r.r2r] = -1;
+            }
+            const bool oob = nc && (((int32_t)r.ord > 63 && r.ord - 64u >= d.W0) ||
+                                    ((int32_t)r.rs1 > 63 && r.rs1 - 64u >= d.W0) ||
+                                    ((int32_t)r.rs2 > 63 && r.rs2 - 64u >= d.W0));
+            if (oob) atomicOr(d.flags, EF_OOB);
+            const bool ok = nc && !oob;
+            for (uint32_t k = 0; k < 8; ++k) { if (ok && lane == k && (int32_t)r.ord > 63) d.sym[r.ord - 64u] = (uint16_t)r.rdr; __syncthreads(); }
+            for (uint32_t k = 0; k < 8; ++k) { if (ok && lane == k && (int32_t)r.rs1 > 63) d.sym[r.rs1 - 64u] = (uint16_t)r.r1r; __syncthreads(); }
+            for (uint32_t k = 0; k < 8; ++k) { if (ok && lane == k && (int32_t)r.rs2 > 63) d.sym[r.rs2 - 64u] = (uint16_t)r.r2r; __syncthreads(); }
+        }
     }
 }
 

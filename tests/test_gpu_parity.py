@@ -117,6 +117,20 @@ def test_gpu_sort_permutation(gpu_ctx, which):
     assert bad.size == 0, f"{bad.size} positions differ, first at {bad[0]}"
 
 
+def test_gpu_register_named_by_two_functions(gpu_ctx):
+    # calls with many arguments: an fmv.x.w at the start of one function names
+    # a register that the end of the previous function defines; the program is
+    # allocated in the reference's exact step/group order (k_ra_serial)
+    t = W.program_of_size((W.SEED_STRESS << 32) + 3, 64000, W.STRESS)
+    say("stress program with shared register names", t.n)
+    gpu_ctx.profile(True)
+    r = gpu_ctx.compile(t)
+    names = gpu_ctx.kernel_times()
+    gpu_ctx.profile(False)
+    assert "k_ra_serial" in names
+    assert_same(r, oracle.compile_tree(t))
+
+
 def test_gpu_one_million_nodes(gpu_ctx):
     t = W.program_of_size(0x5EED0007, 1_000_000)
     say("1M-node program", t.n)
