@@ -78,11 +78,25 @@ int ptl_upload(ptl_ctx *ctx, int slot, const ptl_ast *in);
 int ptl_compile_resident(ptl_ctx *ctx, int slot, int copy_out, ptl_result *out);
 const uint32_t *ptl_result_device(ptl_ctx *ctx);
 
-/* Batched compile: n_programs trees concatenated node-wise; program p covers
- * nodes [prog_node_offsets[p], prog_node_offsets[p+1]) and its parents are
- * local to that range.  out[p] points into the context's result storage. */
+/* Batched compile (no counterpart in the reference, which compiles one
+ * DepthTree per rv_generator): n_programs trees concatenated node-wise; program
+ * p covers nodes [prog_node_offsets[p], prog_node_offsets[p+1]) (offsets[0] = 0,
+ * offsets[n_programs] = concat->n_nodes), its parents are local to that range
+ * and max_depths[p] is its DepthTree::maxDepth().  All programs run through one
+ * pipeline; out[p] (n_programs entries) is exactly what ptl_compile returns for
+ * program p alone, its words and function table pointing into the context's
+ * result storage.  stage_ns, opt_iterations and tie_slots are the whole batch's.
+ * On the device the programs' words lie back to back in program order. */
 int ptl_compile_batch(ptl_ctx *ctx, const ptl_ast *concat, const int64_t *prog_node_offsets,
                       const uint32_t *max_depths, uint32_t n_programs, ptl_result *out);
+
+/* Resident batch: ptl_upload_batch copies a batch into slot `slot` as
+ * ptl_upload does a tree; ptl_compile_resident_batch compiles the batch (or a
+ * single tree) held there into out[0..n_programs).  ptl_compile_resident
+ * refuses a slot holding more than one program (PTL_ERR_ARG). */
+int ptl_upload_batch(ptl_ctx *ctx, int slot, const ptl_ast *concat, const int64_t *prog_node_offsets,
+                     const uint32_t *max_depths, uint32_t n_programs);
+int ptl_compile_resident_batch(ptl_ctx *ctx, int slot, int copy_out, ptl_result *out);
 
 /* Host-side stage dumps for differential testing (stage 1..7 boundary arrays
  * of the last compile): name is one of "types", "rtypes", "node_data", "locs",

@@ -13,7 +13,8 @@ ERRORS = {
 
 # every symbol include/ptl.h declares
 EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_upload", "ptl_compile_resident",
-           "ptl_result_device", "ptl_compile_batch", "ptl_debug_dump", "ptl_debug_stop_after",
+           "ptl_result_device", "ptl_compile_batch", "ptl_upload_batch", "ptl_compile_resident_batch",
+           "ptl_debug_dump", "ptl_debug_stop_after",
            "ptl_debug_exclusive_scan", "ptl_profile_enable", "ptl_profile_get")
 
 
@@ -54,6 +55,10 @@ def bind(lib):
     lib.ptl_compile_batch.argtypes = [C.c_void_p, C.POINTER(PtlAst), C.c_void_p, C.c_void_p, C.c_uint32,
                                       C.POINTER(PtlResult)]
     lib.ptl_compile_batch.restype = C.c_int
+    lib.ptl_upload_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(PtlAst), C.c_void_p, C.c_void_p, C.c_uint32]
+    lib.ptl_upload_batch.restype = C.c_int
+    lib.ptl_compile_resident_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(PtlResult)]
+    lib.ptl_compile_resident_batch.restype = C.c_int
     lib.ptl_debug_dump.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_uint64]
     lib.ptl_debug_dump.restype = C.c_int64
     lib.ptl_debug_stop_after.argtypes = [C.c_void_p, C.c_int]

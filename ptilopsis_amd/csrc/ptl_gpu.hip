@@ -66,6 +66,13 @@ DEV void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Program of element i given the programs' bases (last p with base[p] <= i).
+DEV uint32_t prog_of(const uint32_t *base, uint32_t P, uint32_t i) {
+    uint32_t lo = 0, hi = P;
+    while (hi - lo > 1) { const uint32_t m = (lo + hi) >> 1; if (base[m] <= i) lo = m; else hi = m; }
+    return lo;
+}
+
 DEV uint32_t ffz_gcc(uint64_t v) {
     // include/utils.hpp:25-45 as compiled by GCC: __builtin_ffs(int) of ~val sees 32 bits.
     uint32_t low = (uint32_t)(~v);
@@ -186,12 +193,16 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(Op op, uint64_t n, con
 
 // ----------------------------------------------------------- device state
 struct Dev {
-    // inputs
+    // inputs: P programs concatenated node-wise, parents global
     const uint8_t *pt, *prt;
     const int32_t *par, *depth, *cidx;
     const uint32_t *ndin;
     uint64_t n;
-    uint32_t maxd;
+    uint32_t maxd;           // largest max depth of the programs
+    uint32_t P;
+    const uint32_t *pn;      // node base of each program (P+1)
+    uint32_t *pw;            // word base of each program after isn_gen (P+1)
+    uint32_t *pf;            // function base of each program (P+1)
     // preprocess / isn_cnt
     uint8_t *ty8, *rt8;
     uint32_t *nd;
@@ -211,6 +222,7 @@ struct Dev {
     uint4 *w0_f4;            // per word: rd, rs1, rs2, instruction (one 16-byte load)
     uint16_t *sym;           // per virtual register: assigned register | swapped << 8
     uint32_t *xref;          // per virtual register named from outside its word's function: namer + 1
+    uint32_t *cflag;         // per program: a register is named by two of its functions
     uint32_t *ra_off;
     uint32_t W1;
     uint32_t *w1_instr, *w1_rd, *w1_rs1, *w1_rs2, *w1_jt;
@@ -255,11 +267,14 @@ __global__ __launch_bounds__(256) void k_pre12(Dev d) {
         ty = c_pareas[d.pt[i]]; rt = d.prt[i]; ci = (uint32_t)d.cidx[i]; nd = d.ndin[i];
         classify(ty, rt, ci, nd);
     }
-    uint64_t m = __ballot(in && ty == RV_CALL_ARGS);
-    uint32_t lane = threadIdx.x & 63u;
-    bool block_has_list = ((m >> (lane & ~7u)) & 0xFFull) != 0;
     if (!in) return;
-    if (block_has_list && i > 0) {
+    // does the node's 8-node vector (counted from its program's first node)
+    // hold a call argument list?
+    const uint32_t prg = prog_of(d.pn, d.P, (uint32_t)i);
+    const uint32_t v0 = d.pn[prg] + (((uint32_t)i - d.pn[prg]) & ~7u), v1 = min(v0 + 8u, d.pn[prg + 1]);
+    bool block_has_list = false;
+    for (uint32_t j = v0; j < v1; ++j) block_has_list |= c_pareas[d.pt[j]] == RV_CALL_ARGS;
+    if (block_has_list && (uint32_t)i > d.pn[prg]) {
         uint64_t j = i - 1;
         uint32_t pty = c_pareas[d.pt[j]], prt = d.prt[j], pci = (uint32_t)d.cidx[j], pnd = d.ndin[j];
         classify(pty, prt, pci, pnd);
@@ -280,13 +295,14 @@ __global__ __launch_bounds__(256) void k_pre12(Dev d) {
 __global__ __launch_bounds__(256) void k_pre3(Dev d) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= d.n || d.prt[i] != DT_FLOAT) return;
+    const uint32_t b = d.pn[prog_of(d.pn, d.P, (uint32_t)i)];  // vectors count from the program's first node
     uint64_t cur = i;
     for (;;) {
         int32_t p = d.par[cur];
         if (p < 0) break;
         if ((d.ty8[p] & 0xE0u) != 0xE0u) break;
         d.rt8[p] = DT_FLOAT;
-        if ((cur >> 3) < ((uint64_t)p >> 3)) cur = (uint64_t)p;
+        if (((cur - b) >> 3) < (((uint64_t)p - b) >> 3)) cur = (uint64_t)p;
         else break;
     }
 }
@@ -302,7 +318,7 @@ struct CountOp {
             if (i > 0 && (d.ty8[i - 1] & 0xF4u) == 0xF4u) s += (uint32_t)d.cidx[i - 1] + 1u;
         }
         int32_t p = d.par[i];
-        uint32_t pty = d.ty8[p != -1 ? p : 0];
+        uint32_t pty = d.ty8[p != -1 ? (uint32_t)p : d.pn[prog_of(d.pn, d.P, (uint32_t)i)]];  // root: the program's node 0
         int32_t ci = d.cidx[i];
         bool ife = pty == RV_IF_ELSE;
         if (ci == 0 && (ife || pty == RV_IF)) s += 1;
@@ -335,13 +351,47 @@ struct DeclOp {
     DEV void emit(uint64_t i, uint32_t excl, uint32_t v) const { if (v) d.decl[excl] = (uint32_t)i; }
 };
 
+// Every program's first position starts an insertion-sort leaf.
+__global__ void k_prog_bounds(uint8_t *bound, const uint32_t *pn, uint32_t P) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P) bound[pn[p]] = 1;
+}
+
+// Batched input: parents arrive local to their program's node range.
+__global__ void k_globalize_parents(int32_t *par, uint64_t n, const uint32_t *pn, uint32_t P) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t p = par[i];
+    if (p >= 0) par[i] = p + (int32_t)pn[prog_of(pn, P, (uint32_t)i)];
+}
+
+// Word base of every program: the exclusive count scan at its first node (the
+// location fix does not move a program's first node); total at P.
+__global__ void k_prog_words(Dev d, const uint32_t *total) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > d.P) return;
+    d.pw[p] = p < d.P ? d.loc[d.pn[p]] : *total;
+}
+
+// Function base of every program: functions (decl nodes, in node order) before
+// its first node.
+__global__ void k_prog_funcs(Dev d) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > d.P) return;
+    const uint32_t b = p < d.P ? d.pn[p] : (uint32_t)d.n;
+    uint32_t lo = 0, hi = d.F;  // first decl index >= b
+    while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (d.decl[m] < b) lo = m + 1; else hi = m; }
+    d.pf[p] = lo;
+}
+
 // Function table (:451-515): ids = node data, starts = previous decl location + 6.
 __global__ void k_functab(Dev d) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.F) return;
     uint32_t dn = d.decl[k];
     uint32_t off = 6u + d.loc[dn];
-    uint32_t prev = k ? 6u + d.loc[d.decl[k - 1]] : 0u;
+    const uint32_t prg = prog_of(d.pf, d.P, k);
+    uint32_t prev = k > d.pf[prg] ? 6u + d.loc[d.decl[k - 1]] : d.pw[prg];
     d.fids[k] = d.nd[dn];
     d.fstarts[k] = prev;
     d.fsizes[k] = off - prev;
@@ -352,11 +402,15 @@ __global__ void k_functab(Dev d) {
 struct Lane {
     bool has;
     int32_t pai;
-    uint32_t dpv, loc, word, rd, rs1, rs2, jt;
+    uint32_t dpv, loc, word, rd, rs1, rs2, jt, wb;
 };
 
-// One (node, offset) lane of the level loop body (:585-884).
+// One (node, offset) lane of the level loop body (:585-884), computed in the
+// word indices of the node's own program (register numbers are ORed with word
+// indices, :676-697, :842); L.loc is the global word slot.
 DEV void gen_lane(const Dev &d, uint32_t nn, uint32_t o, bool full, Lane &L) {
+    const uint32_t prg = prog_of(d.pn, d.P, nn);
+    const uint32_t wb = d.pw[prg], fb = d.pf[prg], Fp = d.pf[prg + 1] - fb;
     const uint32_t ty = d.ty8[nn], rt = d.rt8[nn];
     const uint32_t si = spec_i(ty, o, rt);
     const uint32_t b = c_bits[si];
@@ -376,7 +430,7 @@ DEV void gen_lane(const Dev &d, uint32_t nn, uint32_t o, bool full, Lane &L) {
     }
     L.has = has;
     L.pai = call ? (int32_t)((uint32_t)par * 3u + ci) : -1;
-    const uint32_t iib = o + (prop ? d.loc[nn] : 0u);
+    const uint32_t iib = o + (prop ? d.loc[nn] - wb : 0u);
     uint32_t rd = 0;
     const uint32_t ndv = d.nd[nn];
     if (has) {
@@ -413,8 +467,8 @@ DEV void gen_lane(const Dev &d, uint32_t nn, uint32_t o, bool full, Lane &L) {
     const uint32_t jc = (b >> 16) & 7u;
     uint32_t jt = 0;
     if (jc >= 1 && jc <= 5) jt = d.regs[(uint64_t)nn * 3 + 1 + (jc > 2 ? 1 : 0)] + ((jc & 1u) == 0 ? 1u : 0u);
-    if (jc == 6) { if (ndv < d.F) jt |= d.fends[ndv] - 6u; else atomicOr(d.flags, EF_OOB); }
-    if (jc == 7) { if (ndv < d.F) jt |= d.fstarts[ndv]; else atomicOr(d.flags, EF_OOB); }
+    if (jc == 6) { if (ndv < Fp) jt |= d.fends[fb + ndv] - wb - 6u; else atomicOr(d.flags, EF_OOB); }
+    if (jc == 7) { if (ndv < Fp) jt |= d.fstarts[fb + ndv] - wb; else atomicOr(d.flags, EF_OOB); }
     uint32_t rs[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
@@ -432,7 +486,8 @@ DEV void gen_lane(const Dev &d, uint32_t nn, uint32_t o, bool full, Lane &L) {
         if (oc == 7) arg |= iib + 63u;
         rs[s] = arg;
     }
-    L.loc = iloc; L.word = word; L.rd = rd; L.rs1 = rs[0]; L.rs2 = rs[1]; L.jt = jt;
+    L.loc = iloc + wb; L.word = word; L.rd = rd; L.rs1 = rs[0]; L.rs2 = rs[1]; L.jt = jt;
+    L.wb = wb;
 }
 
 // Parent-slot propagation: the last propagating lane of a node wins the slot.
@@ -491,8 +546,18 @@ __global__ __launch_bounds__(256) void k_gen_write(Dev d, int use_rank) {
         gen_lane(d, (uint32_t)i, o, true, L);
         if (!L.has || L.loc >= d.W0) continue;
         if (d.owner[L.loc] != owner_key(d, (uint32_t)i, o, use_rank != 0)) continue;
-        d.w0_instr[L.loc] = L.word; d.w0_rd[L.loc] = L.rd; d.w0_rs1[L.loc] = L.rs1;
-        d.w0_rs2[L.loc] = L.rs2; d.w0_jt[L.loc] = L.jt;
+        // virtual registers (> 63) and targets become global word indices; a
+        // virtual register past its program's words has no state (the
+        // single-program check is virt_ok's)
+        const uint32_t wn = d.pw[prog_of(d.pn, d.P, (uint32_t)i) + 1] - L.wb;
+        if (d.P > 1 && (((int32_t)L.rd > 63 && L.rd - 64u >= wn) || ((int32_t)L.rs1 > 63 && L.rs1 - 64u >= wn) ||
+                        ((int32_t)L.rs2 > 63 && L.rs2 - 64u >= wn)))
+            atomicOr(d.flags, EF_OOB);
+        d.w0_instr[L.loc] = L.word;
+        d.w0_rd[L.loc] = (int32_t)L.rd > 63 ? L.rd + L.wb : L.rd;
+        d.w0_rs1[L.loc] = (int32_t)L.rs1 > 63 ? L.rs1 + L.wb : L.rs1;
+        d.w0_rs2[L.loc] = (int32_t)L.rs2 > 63 ? L.rs2 + L.wb : L.rs2;
+        d.w0_jt[L.loc] = L.jt + L.wb;
     }
 }
 
@@ -533,8 +598,10 @@ DEV int32_t median_to_first(uint32_t first, uint32_t last, uint32_t *sidx, int32
     return skey[first];
 }
 
-__global__ void k_sort_root(const Seg *segs, uint32_t *sidx, int32_t *skey, int32_t *pivot) {
-    pivot[0] = median_to_first(segs[0].first, segs[0].last, sidx, skey);
+// The first level: one range per program of more than 16 nodes.
+__global__ void k_sort_root(const Seg *segs, uint32_t m, uint32_t *sidx, int32_t *skey, int32_t *pivot) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < m) pivot[s] = median_to_first(segs[s].first, segs[s].last, sidx, skey);
 }
 
 // Last range whose active offset is <= a.
@@ -1000,7 +1067,7 @@ DEV void xref_note(const Dev &d, uint32_t v, uint32_t f) {
     const uint32_t e = v - 64u;
     if (e >= d.W0) return;  // out of range: flagged by the caller's bounds checks
     const uint32_t prev = atomicCAS(&d.xref[e], 0u, f + 1u);
-    if (prev != 0u && prev != f + 1u) atomicOr(d.flags, EF_XCONFLICT);
+    if (prev != 0u && prev != f + 1u) { atomicOr(d.flags, EF_XCONFLICT); d.cflag[prog_of(d.pf, d.P, f)] = 1; }
     atomicOr(d.flags, EF_XNAMED);
 }
 
@@ -1038,11 +1105,20 @@ DEV void ra_commit_syms(const Dev &d, const RaStep &r) {
 // threads, consecutive words of one function).
 constexpr uint32_t WALK_CACHE = 64;
 constexpr uint32_t WALK_CHUNK = 32;
-// Workgroups spread their per-step lookup records over WALK_COPIES copies of
-// the (step, register) table (copy = workgroup mod WALK_COPIES), which
+// Workgroups spread their per-step lookup records over up to WALK_COPIES
+// copies of the (step, register) tables (copy = workgroup mod copies), which
 // k_walk_merge folds: one table shared by every workgroup serializes its
 // atomics on the same few addresses.
 constexpr uint32_t WALK_COPIES = 16;
+
+// One program's allocation: functions from fbase (nf of them) in groups of 8
+// (L groups), maxsize = its longest function; its slices of the lane-0 group
+// order (perm + poff) and of the (step, register) lookup tables (+ gbase).  The
+// tables hold `rows` steps: every step while a lane-0 function is running, then
+// (when some other function runs longer) one last row that gathers all later
+// steps.  Past its lane-0 functions the snapshot no longer changes and no owner
+// reads, so a later lookup only marks, whatever its step.
+struct Ra0Prog { uint32_t fbase, L, maxsize, poff; uint64_t gbase; uint32_t nf, rows; };
 
 // The walk's cache of a function's states, kept by one thread: ring slot x & 7
 // holds the newest definitions (tag x << 10 | valid << 9 | state); the LDS
@@ -1083,7 +1159,8 @@ DEV void wc_lds_put(const Dev &d, uint32_t (*c)[64], uint32_t lane, uint32_t sta
 }
 
 __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint32_t *gmax_all, uint64_t stride,
-                                                const uint32_t *permw, uint32_t nw) {
+                                                uint32_t copies, const Ra0Prog *progs, const uint32_t *permw,
+                                                uint32_t nw) {
     __shared__ uint32_t smin[64], smax[64];
     __shared__ uint32_t cache[WALK_CACHE][64];
     __shared__ uint4 words[WALK_CHUNK][64];
@@ -1094,9 +1171,18 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
     const bool act = f < d.F;
     const uint32_t start = act ? d.fstarts[f] : 0u;
     const uint32_t size = act ? d.fsizes[f] : 0u;
-    const uint32_t grp = f >> 3;
-    uint32_t *gmin = gmin_all + (blockIdx.x % WALK_COPIES) * stride;
-    uint32_t *gmax = gmax_all + (blockIdx.x % WALK_COPIES) * stride;
+    // group = function index within its program / 8; lookup table of the program
+    const uint32_t prg = act ? prog_of(d.pf, d.P, f) : 0xFFFFFFFFu;
+    const uint32_t grp = act ? (f - d.pf[prg]) >> 3 : 0u;
+    uint32_t pmin = prg, pmax = act ? prg : 0u;
+    for (int s = 32; s > 0; s >>= 1) { pmin = min(pmin, (uint32_t)__shfl_xor((int)pmin, s, 64)); pmax = max(pmax, (uint32_t)__shfl_xor((int)pmax, s, 64)); }
+    const bool one = pmin == pmax;  // the workgroup's functions all belong to one program
+    uint32_t *gmin = gmin_all + (blockIdx.x % copies) * stride;
+    uint32_t *gmax = gmax_all + (blockIdx.x % copies) * stride;
+    const uint64_t tb = one ? progs[pmin].gbase : 0u;  // table base of that program
+    const uint32_t tlast = one ? progs[pmin].rows - 1u : 0u;
+    const uint64_t mbase = act ? progs[prg].gbase : 0u;  // own program's table and its last row
+    const uint32_t mlast = act ? progs[prg].rows - 1u : 0u;
     const bool con = size < (1u << 22);  // cache on (tags hold 22 bits of word offset)
     WcRing ring{0, 0, 0, 0, 0, 0, 0, 0};
     // state of virtual register v (inside [64, 64 + W0)); then its update
@@ -1146,9 +1232,16 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
             const uint32_t raw2 = virt_ok(d, w.z) ? (get(w.z) & 0xFFu) : 0u;
             const RaStep r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1, raw2, false, false, life);
             xref_check_fields(d, f, start, size, r.ord, r.rs1, r.rs2);
-            if (r.swd) { atomicMin(&smin[r.rdr], grp); atomicMax(&smax[r.rdr], grp + 1u); }
-            if (r.sw1) { atomicMin(&smin[r.r1r], grp); atomicMax(&smax[r.r1r], grp + 1u); }
-            if (r.sw2) { atomicMin(&smin[r.r2r], grp); atomicMax(&smax[r.r2r], grp + 1u); }
+            if (one) {
+                if (r.swd) { atomicMin(&smin[r.rdr], grp); atomicMax(&smax[r.rdr], grp + 1u); }
+                if (r.sw1) { atomicMin(&smin[r.r1r], grp); atomicMax(&smax[r.r1r], grp + 1u); }
+                if (r.sw2) { atomicMin(&smin[r.r2r], grp); atomicMax(&smax[r.r2r], grp + 1u); }
+            } else {  // functions of several programs: post straight to each program's table
+                const uint64_t row = mbase + (uint64_t)min(i, mlast) * 64u;
+                if (r.swd) { atomicMin(&gmin[row + r.rdr], grp); atomicMax(&gmax[row + r.rdr], grp + 1u); }
+                if (r.sw1) { atomicMin(&gmin[row + r.r1r], grp); atomicMax(&gmax[row + r.r1r], grp + 1u); }
+                if (r.sw2) { atomicMin(&gmin[row + r.r2r], grp); atomicMax(&gmax[row + r.r2r], grp + 1u); }
+            }
             life = r.newlife;
             // register_state updates (ra_commit_syms): destination, then rs1, then rs2
             if (((int32_t)r.ord > 63 && r.ord - 64u >= d.W0) || ((int32_t)r.rs1 > 63 && r.rs1 - 64u >= d.W0) ||
@@ -1164,8 +1257,8 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
         lds_barrier();
         uint32_t mx = smax[lane];
         if (mx) {
-            atomicMin(&gmin[(uint64_t)i * 64u + lane], smin[lane]);
-            atomicMax(&gmax[(uint64_t)i * 64u + lane], mx);
+            atomicMin(&gmin[tb + (uint64_t)min(i, tlast) * 64u + lane], smin[lane]);
+            atomicMax(&gmax[tb + (uint64_t)min(i, tlast) * 64u + lane], mx);
             smin[lane] = 0xFFFFFFFFu; smax[lane] = 0;
         }
         lds_barrier();
@@ -1179,12 +1272,12 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
     }
 }
 
-// gmin/gmax[e] = min/max over the WALK_COPIES copies (copy c at c * stride).
-__global__ void k_walk_merge(uint32_t *gmin, uint32_t *gmax, uint64_t stride) {
+// gmin/gmax[e] = min/max over the copies (copy c at c * stride).
+__global__ void k_walk_merge(uint32_t *gmin, uint32_t *gmax, uint64_t stride, uint32_t copies) {
     const uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (e >= stride) return;
     uint32_t mn = gmin[e], mx = gmax[e];
-    for (uint32_t c = 1; c < WALK_COPIES; ++c) {
+    for (uint32_t c = 1; c < copies; ++c) {
         mn = min(mn, gmin[c * stride + e]);
         mx = max(mx, gmax[c * stride + e]);
     }
@@ -1217,13 +1310,17 @@ DEV RaStep ra0_eval(const Dev &d, uint32_t off, uint64_t life, bool f1, bool f2)
     return ra_eval(d, off, life, s1, s2);
 }
 
-__global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_t maxsize, const uint32_t *gmin,
-                                                         const uint32_t *gmax, uint64_t *life0, Ra0Rec *rec) {
+
+__global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, const Ra0Prog *progs, const uint32_t *gmin_all,
+                                                         const uint32_t *gmax_all, uint64_t *life0, Ra0Rec *rec) {
     __shared__ int32_t Rv[64], Ro[64], Sv[64], So[64];
     __shared__ uint32_t Lmin[64], Lmax[64], fbits[64], wkey[64];
     __shared__ int changed;
     const uint32_t tid = threadIdx.x;
-    const uint32_t L = (d.F + 7u) / 8u;
+    const Ra0Prog pg = progs[blockIdx.x];
+    const uint32_t L = pg.L, maxsize = pg.rows, fbase = pg.fbase;
+    const uint32_t *gmin = gmin_all + pg.gbase, *gmax = gmax_all + pg.gbase;
+    life0 += pg.poff; rec += pg.poff;
     if (tid < 64) { Rv[tid] = -1; Ro[tid] = -1; }
     for (uint32_t j = tid; j < L; j += RA0_THREADS) life0[j] = 0x1Full;
     __syncthreads();
@@ -1235,7 +1332,7 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
             if (tid == 0) changed = 0;
             __syncthreads();
             for (uint32_t j = tid; j < L; j += RA0_THREADS) {
-                const uint32_t f = 8u * j;
+                const uint32_t f = fbase + 8u * j;
                 if (!((int32_t)i < (int32_t)d.fsizes[f])) continue;
                 bool f1, f2;
                 ra0_flips(So, fbits, j, f1, f2);
@@ -1249,7 +1346,7 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
                 uint32_t nb = 0;
                 if (Sv[tid] >= 64 && So[tid] >= 0) {
                     const uint32_t o = (uint32_t)So[tid];
-                    const uint32_t f = 8u * o;
+                    const uint32_t f = fbase + 8u * o;
                     if ((int32_t)i < (int32_t)d.fsizes[f] && Lmin[tid] < o) {
                         const uint32_t off = d.fstarts[f] + i;
                         if (d.w0_f4[off].y == (uint32_t)Sv[tid]) nb |= 1u;
@@ -1268,7 +1365,7 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
         }
         // commit: lifetime, records, region-0 write keys (group order: last writer wins)
         for (uint32_t j = tid; j < L; j += RA0_THREADS) {
-            const uint32_t f = 8u * j;
+            const uint32_t f = fbase + 8u * j;
             if (!((int32_t)i < (int32_t)d.fsizes[f])) continue;
             bool f1, f2;
             ra0_flips(So, fbits, j, f1, f2);
@@ -1281,7 +1378,7 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
         }
         __syncthreads();
         for (uint32_t j = tid; j < L; j += RA0_THREADS) {
-            const uint32_t f = 8u * j;
+            const uint32_t f = fbase + 8u * j;
             if (!((int32_t)i < (int32_t)d.fsizes[f])) continue;
             const Ra0Rec q = rec[j];
             RaStep r;
@@ -1299,7 +1396,7 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, uint32_
             bool mark = true;
             if (So[tid] >= 0) {
                 const uint32_t o = (uint32_t)So[tid];
-                const uint32_t f = 8u * o;
+                const uint32_t f = fbase + 8u * o;
                 if ((int32_t)i < (int32_t)d.fsizes[f]) {
                     const Ra0Rec q = rec[o];
                     bool reset = (q.ord == v) || (q.rs1 == v) || (q.rs2 == v);
@@ -1423,11 +1520,11 @@ DEV void ra0_keys(uint32_t *wkey, const RaStep &r, uint32_t j) {
     if (r.r2r != r.rdr) atomicMax(&wkey[r.r2r], ((j + 1u) << 2) | 3u);
 }
 
-template <int PER>
-__global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, const uint32_t *gmin,
-                                                   const uint32_t *gmax, const uint32_t *perm) {
-    __shared__ uint32_t cur_rd[RA0_LDS_FNS], cur_rs1[RA0_LDS_FNS], cur_rs2[RA0_LDS_FNS];
-    __shared__ uint32_t rrs[RA0_LDS_FNS];                     // step result per group: r1r | r2r << 8 | rdr << 16
+template <int PER, uint32_t NL>
+__global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, const uint32_t *gmin_all,
+                                                   const uint32_t *gmax_all, const uint32_t *perm_all) {
+    __shared__ uint32_t cur_rd[NL], cur_rs1[NL], cur_rs2[NL];
+    __shared__ uint32_t rrs[NL];                              // step result per group: r1r | r2r << 8 | rdr << 16
     __shared__ int32_t Sv[2][64], So[2][64];                  // region-0 snapshot of step i / i+1
     __shared__ uint32_t Lmin[2][64], Lmax[2][64];             // lookups by all groups but the owners
     __shared__ uint32_t Omin[2][64], Omax[2][64];             // lookups by the (possible) owners
@@ -1436,7 +1533,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, cons
     __shared__ uint32_t mk[2][2];                             // snapshot entries marked at step i
     __shared__ int changed;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
-    const uint32_t L = (d.F + 7u) / 8u;
+    // this workgroup's program: groups of 8 functions from fbase, L of them
+    const Ra0Prog pg = progs[blockIdx.x];
+    const uint32_t L = pg.L, maxsize = pg.rows, fbase = pg.fbase;  // steps: the table's rows
+    const uint32_t *perm = perm_all + pg.poff;
+    const uint32_t *gmin = gmin_all + pg.gbase, *gmax = gmax_all + pg.gbase;
     const uint32_t cb = nth > 64 ? 64u : 0u;  // first thread of the table-clearing threads
     Ra0Fn fn[PER];
 #pragma unroll
@@ -1446,7 +1547,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, cons
         f.j = 0; f.start = 0; f.size = 0; f.life = 0x1Full; f.own = 0;
         if (t < L) {
             f.j = perm[t];
-            if (f.j < L) { f.start = d.fstarts[8u * f.j]; f.size = d.fsizes[8u * f.j]; }
+            if (f.j < L) { f.start = d.fstarts[fbase + 8u * f.j]; f.size = d.fsizes[fbase + 8u * f.j]; }
             else { atomicOr(d.flags, EF_OOB); f.j = 0; }
         }
         const uint4 w0 = ra0_word(d, f, 0);
@@ -1481,7 +1582,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, cons
             const uint32_t j = f.j;
             if ((int32_t)i < (int32_t)f.size) {
                 ra0_cur(f, cur_rd, cur_rs1, cur_rs2, j);
-                xref_check_fields(d, 8u * j, f.start, f.size, f.ord, f.rs1, f.rs2);
+                xref_check_fields(d, fbase + 8u * j, f.start, f.size, f.ord, f.rs1, f.rs2);
                 const RaStep r = ra0_step(f);
                 if (f.own) {
                     ra0_lookups(Omin[c], Omax[c], r, j);
@@ -1648,23 +1749,29 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, uint32_t maxsize, cons
 // register_state, then the symbol entries of rd, rs1, rs2, lane by lane).
 // register_state keeps 8 regions of 64 (region = lane); lookups read region 0
 // of the step's snapshot.
-__global__ __launch_bounds__(64) void k_ra_serial(Dev d, uint32_t maxsize, uint64_t *life) {
+__global__ __launch_bounds__(64) void k_ra_serial(Dev d, const Ra0Prog *progs, const uint32_t *plist,
+                                                  uint64_t *life_all) {
     __shared__ int32_t reg[8][64];
     __shared__ int32_t snap[64];
     const uint32_t lane = threadIdx.x;
+    const uint32_t prg = plist[blockIdx.x];
+    const Ra0Prog pg = progs[prg];
+    const uint32_t maxsize = pg.maxsize, fbase = pg.fbase, nf = pg.nf;
+    uint64_t *life = life_all + fbase;
     for (uint32_t k = lane; k < 8 * 64; k += 64) reg[k / 64][k % 64] = -1;
-    for (uint32_t f = lane; f < d.F; f += 64) life[f] = 0x1Full;
+    for (uint32_t f = lane; f < nf; f += 64) life[f] = 0x1Full;
+    for (uint32_t e = d.pw[prg] + lane; e < d.pw[prg + 1]; e += 64) d.sym[e] = 0;  // the program's states restart
     __syncthreads();
-    const uint32_t G = (d.F + 7u) / 8u;
+    const uint32_t G = (nf + 7u) / 8u;
     for (uint32_t i = 0; i < maxsize; ++i) {
         snap[lane] = reg[0][lane];
         __syncthreads();
         for (uint32_t g = 0; g < G; ++g) {
-            const uint32_t f = 8u * g + lane;
-            const bool nc = lane < 8 && f < d.F && (int32_t)i < (int32_t)d.fsizes[f];
+            const uint32_t f = 8u * g + lane;  // within the program
+            const bool nc = lane < 8 && f < nf && (int32_t)i < (int32_t)d.fsizes[fbase + f];
             uint4 w = make_uint4(0, 0, 0, 0);
             uint64_t lt = 0;
-            if (nc) { w = d.w0_f4[d.fstarts[f] + i]; lt = life[f]; }
+            if (nc) { w = d.w0_f4[d.fstarts[fbase + f] + i]; lt = life[f]; }
             const uint32_t raw1 = nc && virt_ok(d, w.y) ? d.sym[w.y - 64u] : 0u;
             const uint32_t raw2 = nc && virt_ok(d, w.z) ? d.sym[w.z - 64u] : 0u;
             const RaStep r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1 & 0xFFu, raw2 & 0xFFu, (raw1 >> 8) != 0,
@@ -1706,7 +1813,7 @@ __global__ __launch_bounds__(256) void k_xref_check(Dev d) {
         if (!t) continue;
         uint32_t lo = 0, hi = d.F;  // function of word i: last start <= i
         while (hi - lo > 1) { uint32_t m = (lo + hi) / 2; if (d.fstarts[m] <= i) lo = m; else hi = m; }
-        if (lo + 1u != t) atomicOr(d.flags, EF_XCONFLICT);
+        if (lo + 1u != t) { atomicOr(d.flags, EF_XCONFLICT); d.cflag[prog_of(d.pf, d.P, lo)] = 1; }
     }
 }
 
@@ -1777,14 +1884,28 @@ __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
     }
 }
 
-// fix_func_tab (:1816-1837) against an offset array of `len` entries.
-__global__ void k_fixtab(Dev d, const uint32_t *off, uint32_t len) {
+// fix_func_tab (:1816-1837) against an offset array; pend[p] is the end of
+// program p's range of that array (a function ending there takes the last
+// offset + 1, as the reference does at the end of its array).
+__global__ void k_fixtab(Dev d, const uint32_t *off, const uint32_t *pend) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.F) return;
+    const uint32_t len = pend[prog_of(d.pf, d.P, k)];
     uint32_t s = d.fstarts[k], e = s + d.fsizes[k];
     uint32_t so = off[s];
     uint32_t eo = (int32_t)e >= (int32_t)len ? off[len - 1] + 1u : off[e];
     d.fstarts[k] = so; d.fends[k] = eo; d.fsizes[k] = eo - so;
+}
+
+// Per-program bounds after regalloc (the reference's W1 = last offset + 1,
+// :1542) and after fix_jumps (last offset + size of the last word, :1891):
+// b1[p] = start of program p's allocated words, e1[p] = its W1 end; b2/e2 the
+// same for the final words.
+__global__ void k_prog_w1(Dev d, uint32_t *b1, uint32_t *e1) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    b1[p] = d.ra_off[d.pw[p]];
+    e1[p] = d.ra_off[d.pw[p + 1] - 1] + 1u;
 }
 
 // Frame words (:1714-1811): lui s0, 0 / addi s0, s0, 8 after the prologue
@@ -1807,6 +1928,13 @@ __global__ void k_ra_frame(Dev d) {
 DEV uint32_t fj_size(uint32_t w) {
     bool jmp = ((w & 0x7Fu) == 0x67u) && w != 0x8067u;
     return jmp ? 2u : (w != 0 ? 1u : 0u);
+}
+
+__global__ void k_prog_w2(Dev d, const uint32_t *b1, const uint32_t *e1, uint32_t *b2, uint32_t *e2) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    b2[p] = d.fj_off[b1[p]];
+    e2[p] = d.fj_off[e1[p] - 1] + fj_size(d.w1_instr[e1[p] - 1]);
 }
 
 struct FjSizeOp {

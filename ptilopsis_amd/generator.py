@@ -66,6 +66,8 @@ class Context:
         s = self.lib.ptl_upload(self.h, slot, C.byref(a))
         if s != abi.PTL_OK:
             raise PtlError(s)
+        self._slot_programs = getattr(self, "_slot_programs", {})
+        self._slot_programs[slot] = 1
 
     def compile_resident(self, slot: int, copy_out: bool = True) -> dict:
         r = abi.PtlResult()
@@ -73,6 +75,44 @@ class Context:
         if s != abi.PTL_OK:
             raise PtlError(s)
         return result_dict(r, copy_out)
+
+    @staticmethod
+    def _batch(trees):
+        """Concatenated arrays (parents stay local to each program), the i64
+        program node offsets and the per-program max depths."""
+        arrs = [np.concatenate([np.ascontiguousarray(t.arrays()[k]) for t in trees]) for k in range(6)]
+        offs = np.zeros(len(trees) + 1, np.int64)
+        offs[1:] = np.cumsum([t.n for t in trees])
+        md = np.array([t.max_depth for t in trees], np.uint32)
+        a = abi.PtlAst(*[x.ctypes.data for x in arrs], int(offs[-1]), int(md.max()))
+        return a, (arrs, offs, md)
+
+    def compile_batch(self, trees, copy_out: bool = True) -> list[dict]:
+        """ptl_compile_batch: every program through one pipeline; one result per program."""
+        a, keep = self._batch(trees)
+        _, offs, md = keep
+        out = (abi.PtlResult * len(trees))()
+        s = self.lib.ptl_compile_batch(self.h, C.byref(a), offs.ctypes.data, md.ctypes.data, len(trees), out)
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return [result_dict(out[p], copy_out) for p in range(len(trees))]
+
+    def upload_batch(self, slot: int, trees) -> None:
+        a, keep = self._batch(trees)
+        _, offs, md = keep
+        s = self.lib.ptl_upload_batch(self.h, slot, C.byref(a), offs.ctypes.data, md.ctypes.data, len(trees))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        self._slot_programs = getattr(self, "_slot_programs", {})
+        self._slot_programs[slot] = len(trees)
+
+    def compile_resident_batch(self, slot: int, copy_out: bool = True) -> list[dict]:
+        n = getattr(self, "_slot_programs", {}).get(slot, 1)
+        out = (abi.PtlResult * n)()
+        s = self.lib.ptl_compile_resident_batch(self.h, slot, 1 if copy_out else 0, out)
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return [result_dict(out[p], copy_out) for p in range(n)]
 
     def profile(self, on: bool = True) -> None:
         self.lib.ptl_profile_enable(self.h, 1 if on else 0)
