@@ -8,13 +8,20 @@ HBM.  `value` = nodes * steps / wall time of the timed steps.
 
 The JSON line also carries:
   roofline      the dominant kernel's algorithmic bytes per launch over its
-                HIP-event duration on the library's stream, against 8 TB/s;
+                HIP-event duration on the library's stream, against 8 TB/s,
+                and (latency) the allocation walk's step count times its
+                measured time per step: the walk is a chain of dependent steps;
   cpu_baseline  the CPU oracle (a one-thread restatement of the reference
                 generator) timed on the same input on this host, whose output
                 also checks the GPU result word for word (bit_exact);
-  stages_ms     per-stage device time.
+  stages_ms     per-stage device time;
+  batched       the same metric for the batched pipeline on batch_4096 (4096
+                programs, ~100M nodes, one pass) and stress_1024 (1024
+                register/jump stress programs of ~64k nodes), each with
+                sampled programs checked word for word against the oracle.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large_16m]
+                    [--batch-steps K] [--no-batch] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -63,6 +70,48 @@ def kernel_alg_bytes(name: str, n: int, w0: int, w1: int, w2: int, f: int) -> fl
     return table.get(name)
 
 
+def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: int) -> dict:
+    """Batched pipeline over one named batch workload, resident in HBM."""
+    import oracle
+    from ptilopsis_amd import workloads as W
+    t0 = time.time()
+    trees = W.batch_4096() if name == "batch_4096" else W.stress_1024()
+    nodes = sum(t.n for t in trees)
+    log(f"{name}: {len(trees)} programs, {nodes} nodes (generated+parsed in {time.time() - t0:.1f}s)")
+    ctx.upload_batch(slot, trees)
+    for i in range(warmup):
+        ctx.compile_resident_batch(slot, copy_out=False)
+    t1 = time.perf_counter()
+    for i in range(steps):
+        ctx.compile_resident_batch(slot, copy_out=False)
+    el = time.perf_counter() - t1
+    ctx.profile(True)
+    res = ctx.compile_resident_batch(slot, copy_out=True)
+    kt = ctx.kernel_times()
+    ctx.profile(False)
+    value = nodes * steps / el
+    log(f"{name}: {el * 1e3 / steps:.1f} ms/step, {value / 1e6:.1f} M nodes/s")
+    log("kernels (ms total, launches): " + ", ".join(
+        f"{k}={v[0] / 1e6:.3f}/{v[1]}" for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:12]))
+    rng = np.random.default_rng(0x5EED)
+    pick = sorted(set(rng.choice(len(trees), min(n_check, len(trees)), replace=False).tolist()
+                      + [int(np.argmax([t.n for t in trees]))]))
+    ok = True
+    for p in pick:
+        want = oracle.compile_tree(trees[p])
+        ok &= all(np.array_equal(res[p][k], want[k]) for k in ("words", "func_ids", "func_starts", "func_ends",
+                                                                  "func_sizes"))
+    log(f"{name}: {len(pick)} sampled programs bit-exact: {ok}")
+    stages = {k: v / 1e6 for k, v in zip(("preprocess", "isn_cnt", "isn_gen", "optimize", "regalloc", "fix_jumps",
+                                          "postprocess"), res[0]["stage_ns"])}
+    out = {"workload": name, "value": value, "unit": "nodes/s", "ms_per_step": el * 1e3 / steps, "steps": steps,
+           "programs": len(trees), "nodes": nodes, "words": int(sum(r["n_words"] for r in res)),
+           "sampled_bit_exact": ok, "sampled_programs": pick, "stages_ms": stages,
+           "top_kernels_ms": {k: v[0] / 1e6 for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:6]}}
+    del trees, res
+    return out
+
+
 def build_workload(name: str):
     from ptilopsis_amd import workloads as W
     if name == "large_16m":
@@ -79,7 +128,22 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="large_16m")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy kernel of BYTES bytes three times (to scale "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 4
+        a = torch.arange(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n, "launches": 3}), flush=True)
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: the code generator has no multi-GPU path "
               "(one program per device; see DESIGN.md for how it would shard)", file=sys.stderr)
@@ -136,6 +200,11 @@ def main() -> int:
                         ("words", "func_ids", "func_starts", "func_ends", "func_sizes"))
         log(f"cpu oracle: {c1 - c0:.1f}s; bit-exact: {bit_exact}")
 
+    batched = None
+    if not args.no_batch:
+        batched = [measure_batch(ctx, nm, 1 + k, args.batch_steps, 1, 8)
+                   for k, nm in enumerate(("batch_4096", "stress_1024"))]
+
     line = {
         "metric": METRIC, "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
@@ -144,6 +213,7 @@ def main() -> int:
                    "max_depth": tree.max_depth, "words": res["n_words"], "path": "plain one-program GPU pipeline",
                    "inputs": "resident in HBM"},
         "roofline": roof, "cpu_baseline": cpu, "bit_exact": bit_exact, "stages_ms": stages,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

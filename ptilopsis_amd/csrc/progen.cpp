@@ -180,7 +180,31 @@ struct Gen {
         scopes.pop_back();
     }
 
-    void statement(uint32_t lvl, uint32_t nest) {
+    // A comparison of depth 2 (the nesting, not the condition, is the point).
+    void small_cond() {
+        static const char *ops[] = { " == ", " != ", " < ", " > ", " <= ", " >= " };
+        Ty ct = pick_ty();
+        binop(ops[rng.next() % 6], ct, 2, ct, 1);
+    }
+
+    // k nested if/while levels, each holding one plain statement and the next
+    // level: long chains of branches and jumps for fix_jumps.
+    void chain(uint32_t lvl, uint32_t k) {
+        if (k == 0) { statement(lvl, P.max_nest, false); return; }
+        indent(lvl);
+        if (rng.coin(0.5)) { out += "while "; nodes += 3; }  // WHILE + WHILE_DUMMY + body STATEMENT_LIST
+        else { out += "if "; nodes += 2; }                    // IF + then STATEMENT_LIST
+        small_cond();
+        out += " {\n";
+        scopes.emplace_back();
+        statement(lvl + 1, P.max_nest, false);
+        chain(lvl + 1, k - 1);
+        scopes.pop_back();
+        indent(lvl); out += "}\n";
+    }
+
+    void statement(uint32_t lvl, uint32_t nest, bool top = false) {
+        if (top && P.chain_max && rng.coin(P.p_chain)) { chain(lvl, rng.range(P.chain_min, P.chain_max)); return; }
         indent(lvl);
         double r = rng.uni();
         bool can_nest = nest < P.max_nest;
@@ -265,7 +289,7 @@ struct Gen {
         out += " {\n";
         if (target) {
             scopes.emplace_back();
-            while (nodes - start < target) statement(1, 0);
+            while (nodes - start < target) statement(1, 0, true);
             scopes.pop_back();
         } else {
             body(1, rng.range(P.min_stmts, P.max_stmts), 0);
@@ -281,7 +305,10 @@ extern "C" int ptl_generate_source(const ptl_gen_params *p, const uint32_t *targ
     Gen g(*p);
     g.out.reserve(targets ? 64 * 1024 : 1 << 20);
     g.nodes = 1;  // root STATEMENT_LIST
-    for (uint32_t f = 0; f < p->n_functions; ++f) g.function(f, targets ? targets[f] : 0);
+    for (uint32_t f = 0; f < p->n_functions; ++f) {
+        if (p->total_target && g.nodes >= p->total_target) break;
+        g.function(f, targets ? targets[f] : 0);
+    }
     char *buf = (char *)malloc(g.out.size() + 1);
     if (!buf) return 1;
     memcpy(buf, g.out.data(), g.out.size());

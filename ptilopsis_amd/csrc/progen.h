@@ -20,6 +20,9 @@ typedef struct {
     double float_frac;                  /* share of float-typed variables and operands */
     uint32_t balanced;                  /* 1: wide balanced expressions */
     uint32_t balanced_min, balanced_max;/* their depth range */
+    uint32_t chain_min, chain_max;      /* 0, or the depth range of nested if/while chains */
+    double p_chain;                     /* a top-level statement is such a chain */
+    uint64_t total_target;              /* 0, or stop adding functions once the program has this many nodes */
 } ptl_gen_params;
 
 /* Generates one program.  `targets` (may be NULL) gives per-function node-count

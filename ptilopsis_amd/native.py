@@ -55,6 +55,9 @@ class GenParams(C.Structure):
         ("float_frac", C.c_double),
         ("balanced", C.c_uint32),
         ("balanced_min", C.c_uint32), ("balanced_max", C.c_uint32),
+        ("chain_min", C.c_uint32), ("chain_max", C.c_uint32),
+        ("p_chain", C.c_double),
+        ("total_target", C.c_uint64),
     ]
 
 

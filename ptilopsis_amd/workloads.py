@@ -15,7 +15,9 @@ large_16m      one program, ~16M nodes, 20k functions.  Per-function sizes are
 batch_4096     4096 programs, log-normal sizes (median 8k, sigma 1.5) clipped to
                [1k, 1M], ~100M nodes in total.
 stress_1024    1024 programs of ~64k nodes: 6-16 mixed args, wide balanced
-               expressions of depth 10-14, if/while nesting 8-16, ~30 % floats.
+               expressions of depth 10-14, chains of if/while nested 8-16 deep,
+               ~30 % floats.  Functions (of ~1.5k-node targets) are added until
+               the program reaches 64k nodes.
 serve_stream   request sizes log-normal (median 4k, sigma 2.0) clipped to [200, 4M].
 """
 from __future__ import annotations
@@ -38,11 +40,17 @@ BASE = dict(
     p_assign=0.50, p_if=0.20, p_while=0.15, p_return=0.10,
     p_var_leaf=0.5, expr_p=0.25, p_leaf_operand=0.75, p_arith=0.55, p_compare=0.30, float_frac=0.5,
     balanced=0, balanced_min=0, balanced_max=0,
+    chain_min=0, chain_max=0, p_chain=0.0, total_target=0,
 )
 
 STRESS = dict(BASE, min_args=6, max_args=16, nested_min=1, nested_max=2, max_nest=16,
               p_assign=0.45, p_if=0.22, p_while=0.18, p_return=0.10, float_frac=0.3,
               balanced=1, balanced_min=10, balanced_max=14)
+
+
+# float_frac 0.08: float operands of int comparisons make whole float subtrees,
+# so ~30 % of the nodes end up float-typed
+STRESS_NESTED = dict(STRESS, chain_min=8, chain_max=16, p_chain=0.35, float_frac=0.08)
 
 
 def _params(seed: int, n_functions: int, base=BASE, **kw) -> dict:
@@ -101,5 +109,14 @@ def batch_4096(n_programs: int = 4096, seed: int = SEED_BATCH) -> list[Tree]:
     return [program_of_size((seed << 32) + p, int(s)) for p, s in enumerate(sizes)]
 
 
+def stress_program(seed: int, nodes: int = 64_000) -> Tree:
+    """Functions of ~1.5k-node targets until the program holds 84 % of `nodes`
+    nodes; the last function's overshoot brings the mean to about `nodes`."""
+    cap = nodes // 64 + 16
+    src, _ = program_source(seed, cap, np.full(cap, 1500, np.uint32), STRESS_NESTED,
+                            total_target=int(nodes * 0.84))
+    return parse_source(src)
+
+
 def stress_1024(n_programs: int = 1024, seed: int = SEED_STRESS, nodes: int = 64_000) -> list[Tree]:
-    return [program_of_size((seed << 32) + p, nodes, STRESS) for p in range(n_programs)]
+    return [stress_program((seed << 32) + p, nodes) for p in range(n_programs)]
