@@ -814,7 +814,10 @@ __global__ __launch_bounds__(256) void k_sort_swap(const Seg *segs, uint32_t m, 
     }
 }
 
-struct SortCnt { uint32_t m, A, nheap, nuni, uni_total, pad[3]; };
+// Level counters.  qa = next-level ranges << 32 | their active elements (reset
+// per level); ue = all-equal ranges << 32 | their elements (accumulated); one
+// 64-bit atomic per pair keeps a list's slots and offsets in the same order.
+struct SortCnt { unsigned long long qa, ue; uint32_t nheap, pad[3]; };
 
 DEV uint32_t block_excl_scan(uint32_t v, uint32_t *tmp /* 16 */, uint32_t &total) {
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
@@ -828,52 +831,53 @@ DEV uint32_t block_excl_scan(uint32_t v, uint32_t *tmp /* 16 */, uint32_t &total
     return pre + x - v;
 }
 
-// One workgroup: the children of every partitioned range, in position order:
-// next level (with their active offsets), heapsort list, or leaves (marked in
+// The children of every partitioned range (1024 ranges per workgroup): next
+// level (with their active offsets), heapsort list, or leaves (marked in
 // `bound`); all-equal ranges to the closed-form list (offsets in its own
-// element order); then median-of-three for every range of the next level.
+// element order); then median-of-three for every new range of the next level.
+// Workgroups take their slots of each list with one atomic, so the lists are
+// in workgroup-arrival order; every consumer needs only that a list's slots and
+// offsets agree.
 __global__ __launch_bounds__(1024) void k_sort_plan(const Seg *segs, const SegCut *cuts, uint32_t m, Seg *next,
                                                     int32_t *next_pivot, Seg *heap, Seg *uni, SortCnt *cnt,
                                                     uint8_t *bound, uint32_t *sidx, int32_t *skey) {
     __shared__ uint32_t tmp[16];
-    uint32_t q0 = 0, a0 = 0, h0 = cnt->nheap, u0 = cnt->nuni, e0 = cnt->uni_total;
-    for (uint32_t b = 0; b < m; b += blockDim.x) {
-        const uint32_t s = b + threadIdx.x;
-        Seg g{0, 0, 0, 0};
-        SegCut ct{};
-        if (s < m) { g = segs[s]; ct = cuts[s]; }
-        const bool v = s < m, un = v && ct.uni;
-        const uint32_t lim = g.limit - 1;
-        const uint32_t c0 = ct.cut - g.first, c1 = g.last - ct.cut;
-        const bool n0 = v && !un && c0 > 16 && lim > 0, n1 = v && !un && c1 > 16 && lim > 0;
-        const bool h0b = v && !un && c0 > 16 && lim == 0, h1b = v && !un && c1 > 16 && lim == 0;
-        uint32_t tq, ta, th, tu, te;
-        const uint32_t xq = block_excl_scan((n0 ? 1u : 0u) + (n1 ? 1u : 0u), tmp, tq);
-        const uint32_t xa = block_excl_scan((n0 ? c0 : 0u) + (n1 ? c1 : 0u), tmp, ta);
-        const uint32_t xh = block_excl_scan((h0b ? 1u : 0u) + (h1b ? 1u : 0u), tmp, th);
-        const uint32_t xu = block_excl_scan(un ? 1u : 0u, tmp, tu);
-        const uint32_t xe = block_excl_scan(un ? g.last - g.first : 0u, tmp, te);
-        if (v) {
-            if (un) {
-                uni[u0 + xu] = Seg{g.first, g.last, g.limit, e0 + xe};
-                bound[g.first] = 2;
-            } else {
-                bound[ct.cut] = 1;
-                uint32_t q = q0 + xq, a = a0 + xa, h = h0 + xh;
-                if (n0) { next[q++] = Seg{g.first, ct.cut, lim, a}; a += c0; }
-                if (n1) { next[q++] = Seg{ct.cut, g.last, lim, a}; }
-                if (h0b) heap[h++] = Seg{g.first, ct.cut, 0, 0};
-                if (h1b) heap[h++] = Seg{ct.cut, g.last, 0, 0};
-            }
-        }
-        q0 += tq; a0 += ta; h0 += th; u0 += tu; e0 += te;
+    __shared__ uint32_t base[5];
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    Seg g{0, 0, 0, 0};
+    SegCut ct{};
+    if (s < m) { g = segs[s]; ct = cuts[s]; }
+    const bool v = s < m, un = v && ct.uni;
+    const uint32_t lim = g.limit - 1;
+    const uint32_t c0 = ct.cut - g.first, c1 = g.last - ct.cut;
+    const bool n0 = v && !un && c0 > 16 && lim > 0, n1 = v && !un && c1 > 16 && lim > 0;
+    const bool h0b = v && !un && c0 > 16 && lim == 0, h1b = v && !un && c1 > 16 && lim == 0;
+    uint32_t tq, ta, th, tu, te;
+    const uint32_t xq = block_excl_scan((n0 ? 1u : 0u) + (n1 ? 1u : 0u), tmp, tq);
+    const uint32_t xa = block_excl_scan((n0 ? c0 : 0u) + (n1 ? c1 : 0u), tmp, ta);
+    const uint32_t xh = block_excl_scan((h0b ? 1u : 0u) + (h1b ? 1u : 0u), tmp, th);
+    const uint32_t xu = block_excl_scan(un ? 1u : 0u, tmp, tu);
+    const uint32_t xe = block_excl_scan(un ? g.last - g.first : 0u, tmp, te);
+    if (threadIdx.x == 0) {
+        const unsigned long long qa = (tq | ta) ? atomicAdd(&cnt->qa, ((unsigned long long)tq << 32) | ta) : 0ull;
+        const unsigned long long ue = (tu | te) ? atomicAdd(&cnt->ue, ((unsigned long long)tu << 32) | te) : 0ull;
+        base[0] = (uint32_t)(qa >> 32); base[1] = (uint32_t)qa;
+        base[2] = th ? atomicAdd(&cnt->nheap, th) : 0u;
+        base[3] = (uint32_t)(ue >> 32); base[4] = (uint32_t)ue;
     }
     __syncthreads();
-    if (threadIdx.x == 0) { cnt->m = q0; cnt->A = a0; cnt->nheap = h0; cnt->nuni = u0; cnt->uni_total = e0; }
-    __threadfence_block();
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < q0; q += blockDim.x)
-        next_pivot[q] = median_to_first(next[q].first, next[q].last, sidx, skey);
+    if (!v) return;
+    if (un) {
+        uni[base[3] + xu] = Seg{g.first, g.last, g.limit, base[4] + xe};
+        bound[g.first] = 2;
+        return;
+    }
+    bound[ct.cut] = 1;
+    uint32_t q = base[0] + xq, a = base[1] + xa, h = base[2] + xh;
+    if (n0) { next[q] = Seg{g.first, ct.cut, lim, a}; next_pivot[q] = median_to_first(g.first, ct.cut, sidx, skey); ++q; a += c0; }
+    if (n1) { next[q] = Seg{ct.cut, g.last, lim, a}; next_pivot[q] = median_to_first(ct.cut, g.last, sidx, skey); }
+    if (h0b) heap[h++] = Seg{g.first, ct.cut, 0, 0};
+    if (h1b) heap[h++] = Seg{ct.cut, g.last, 0, 0};
 }
 
 DEV bool KLESS(const int32_t *skey, uint32_t a, uint32_t b) { return skey[a] < skey[b]; }
