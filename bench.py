@@ -18,7 +18,10 @@ The JSON line also carries:
   batched       the same metric for the batched pipeline on batch_4096 (4096
                 programs, ~100M nodes, one pass) and stress_1024 (1024
                 register/jump stress programs of ~64k nodes), each with
-                sampled programs checked word for word against the oracle.
+                sampled programs checked word for word against the oracle;
+  serving       the request loop (ptl_server) under Poisson arrivals at each
+                rate: p50/p99 submit-to-result latency, nodes/s, batches, and
+                whether every result equals the program compiled alone.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large_16m]
                     [--batch-steps K] [--no-batch] [--no-cpu-baseline]
@@ -70,6 +73,22 @@ def kernel_alg_bytes(name: str, n: int, w0: int, w1: int, w2: int, f: int) -> fl
     return table.get(name)
 
 
+def measured_traffic(kernel: str, workload: str) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    of `workload` (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].split("<")[0] == kernel:
+            try:
+                return float(cells[4].replace(",", ""))
+            except ValueError:
+                return None
+    return None
+
+
 def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: int) -> dict:
     """Batched pipeline over one named batch workload, resident in HBM."""
     import oracle
@@ -112,6 +131,66 @@ def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: i
     return out
 
 
+def measure_serving(rates, seconds: float, pool_n: int) -> dict:
+    """Serving stream: Poisson arrivals at each rate for `seconds`, requests
+    drawn in order from a pool of programs of log-normal size, submitted to the
+    library's request loop (ptl_server) from a host thread as they arrive.  Each
+    result is checked against the same program compiled alone."""
+    import threading
+    from ptilopsis_amd import workloads as W
+    from ptilopsis_amd.generator import Context, Server
+    t0 = time.time()
+    pool = W.serve_pool(pool_n)
+    log(f"serving pool: {pool_n} programs, {sum(t.n for t in pool)} nodes (generated+parsed in {time.time() - t0:.1f}s)")
+    ctx = Context(0)
+    key = lambda r: hash(r["words"].tobytes() + r["func_starts"].tobytes() + r["func_ids"].tobytes())
+    ref = [key(ctx.compile(t)) for t in pool]
+    ctx.close()
+    srv = Server(0, hbm_budget=48 << 30, max_batch=4096, workers=2)
+    for k in [srv.submit(t) for t in pool[:64]]:  # workers' arenas warm
+        srv.wait(k)
+    out = []
+    for rate in rates:
+        arr = W.arrivals(rate, seconds)
+        tickets = [None] * len(arr)
+
+        def submitter():
+            tb = time.perf_counter()
+            for i, ta in enumerate(arr):
+                while time.perf_counter() - tb < ta:
+                    time.sleep(0.00005)
+                tickets[i] = srv.submit(pool[i % pool_n])
+
+        st0 = srv.stats()
+        th = threading.Thread(target=submitter)
+        tb = time.perf_counter()
+        th.start()
+        lat, qlat, ok, nodes = [], [], True, 0
+        for i in range(len(arr)):
+            while tickets[i] is None:
+                time.sleep(0.00005)
+            r, tm = srv.wait(tickets[i])
+            lat.append(tm["total_ns"])
+            qlat.append(tm["queue_ns"])
+            ok &= key(r) == ref[i % pool_n]
+            nodes += pool[i % pool_n].n
+        th.join()
+        el = time.perf_counter() - tb
+        st1 = srv.stats()
+        nb = st1["batches"] - st0["batches"]
+        rec = {"rate_offered": rate, "requests": len(arr), "seconds": el,
+               "rate_achieved": len(arr) / max(1e-9, float(arr[-1]) if len(arr) else 1.0),
+               "p50_ms": float(np.percentile(lat, 50)) / 1e6, "p99_ms": float(np.percentile(lat, 99)) / 1e6,
+               "p99_queue_ms": float(np.percentile(qlat, 99)) / 1e6, "nodes_per_s": nodes / el,
+               "batches": nb, "mean_batch": len(arr) / max(1, nb), "same_as_alone": bool(ok)}
+        log(f"serving {rate} req/s: p50 {rec['p50_ms']:.2f} ms, p99 {rec['p99_ms']:.2f} ms, "
+            f"{rec['nodes_per_s'] / 1e6:.1f} M nodes/s, {nb} batches (mean {rec['mean_batch']:.1f}), same as alone: {ok}")
+        out.append(rec)
+    srv.close()
+    return {"pool_programs": pool_n, "pool_nodes": sum(t.n for t in pool), "seconds_per_rate": seconds,
+            "workers": 2, "rates": out}
+
+
 def build_workload(name: str):
     from ptilopsis_amd import workloads as W
     if name == "large_16m":
@@ -130,6 +209,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", default="500,2000,5000,20000")
+    ap.add_argument("--serve-seconds", type=float, default=2.5)
+    ap.add_argument("--serve-pool", type=int, default=384)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy kernel of BYTES bytes three times (to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
@@ -181,8 +264,15 @@ def main() -> int:
     avg_ns = dns / max(1, dlaunch)
     achieved = (alg / (avg_ns * 1e-9) / 1e9) if alg else None
     roof = {"kernel": dname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": measured_traffic(dname, args.workload),
             "avg_launch_us": avg_ns / 1e3, "launches": dlaunch, "alg_bytes_per_launch": alg}
+    # the allocation lockstep is a chain of dependent steps: its second bound is
+    # steps x time per step
+    rows = int(ctx.dump("ra_rows").max())
+    lane0 = ktimes.get("k_ra_lane0", (0, 1))
+    roof["latency"] = {"kernel": "k_ra_lane0", "steps": rows, "maxsize_words": int(ctx.dump("ra_maxsize").max()),
+                       "us_per_step": lane0[0] / 1e3 / max(1, rows)}
     log("kernels (ms total, launches): " + ", ".join(
         f"{k}={v[0] / 1e6:.3f}/{v[1]}" for k, v in sorted(ktimes.items(), key=lambda kv: -kv[1][0])))
 
@@ -205,6 +295,11 @@ def main() -> int:
         batched = [measure_batch(ctx, nm, 1 + k, args.batch_steps, 1, 8)
                    for k, nm in enumerate(("batch_4096", "stress_1024"))]
 
+    serving = None
+    if not args.no_serve:
+        serving = measure_serving([float(x) for x in args.serve_rates.split(",")], args.serve_seconds,
+                                  args.serve_pool)
+
     line = {
         "metric": METRIC, "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
@@ -213,7 +308,7 @@ def main() -> int:
                    "max_depth": tree.max_depth, "words": res["n_words"], "path": "plain one-program GPU pipeline",
                    "inputs": "resident in HBM"},
         "roofline": roof, "cpu_baseline": cpu, "bit_exact": bit_exact, "stages_ms": stages,
-        "batched": batched,
+        "batched": batched, "serving": serving,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -98,11 +98,42 @@ int ptl_upload_batch(ptl_ctx *ctx, int slot, const ptl_ast *concat, const int64_
                      const uint32_t *max_depths, uint32_t n_programs);
 int ptl_compile_resident_batch(ptl_ctx *ctx, int slot, int copy_out, ptl_result *out);
 
+/* Serving (no counterpart in the reference, whose only caller compiles one
+ * file per process, src/main.cpp:121-157): a request loop over the batched
+ * pipeline.  `workers` threads each own a context (HIP stream + pooled arena)
+ * and repeatedly take every queued request that fits the HBM budget still free
+ * (at most max_batch), compile them as one batch and return the results, so
+ * requests arriving while batches run join the next batch and up to `workers`
+ * batches are on the device at once.  A request's device footprint is
+ * estimated as 1 MiB + 384 B per node; one larger than the whole budget is
+ * refused (PTL_ERR_ADMISSION).  hbm_budget_bytes 0 means 64 GiB.
+ * ptl_server_submit copies the arrays and returns at once; ptl_server_wait
+ * blocks until the request is done and fills out (valid until
+ * ptl_server_release of that ticket) and, if given, its times.  Each result is
+ * exactly what ptl_compile returns for that request alone.  Requests still
+ * queued at ptl_server_destroy are dropped; no wait may be pending then. */
+typedef struct ptl_server ptl_server;
+typedef struct {
+    uint64_t queue_ns;            /* submit to the start of its batch */
+    uint64_t total_ns;            /* submit to results ready */
+    uint32_t batch_programs;      /* programs in the batch it ran in */
+} ptl_serve_times;
+int ptl_server_create(ptl_server **out, int hip_device, uint64_t hbm_budget_bytes, uint32_t max_batch,
+                      uint32_t workers);
+void ptl_server_destroy(ptl_server *s);
+int ptl_server_submit(ptl_server *s, const ptl_ast *in, uint64_t *ticket);
+int ptl_server_wait(ptl_server *s, uint64_t ticket, ptl_result *out, ptl_serve_times *times);
+int ptl_server_release(ptl_server *s, uint64_t ticket);
+int ptl_server_stats(ptl_server *s, uint64_t *batches, uint64_t *requests_done, uint64_t *nodes_done,
+                     uint64_t *largest_batch);
+
 /* Host-side stage dumps for differential testing (stage 1..7 boundary arrays
  * of the last compile): name is one of "types", "rtypes", "node_data", "locs",
  * "func_ids", "func_starts", "func_ends", "func_sizes", "instr", "rd", "rs1",
- * "rs2", "jt", "used", "sym_reg", "sym_sw", "sorted".  Copies up to cap
- * elements (u32) into dst and returns the element count, or a negative status. */
+ * "rs2", "jt", "used", "sym_reg", "sym_sw", "sorted", and per program
+ * "ra_rows" (steps of the allocation lockstep) and "ra_maxsize" (its longest
+ * function, in words).  Copies up to cap elements (u32) into dst and returns
+ * the element count, or a negative status. */
 int64_t ptl_debug_dump(ptl_ctx *ctx, const char *name, uint32_t *dst, uint64_t cap);
 
 /* Runs the pipeline only up to and including `stage` (1..7) on the next

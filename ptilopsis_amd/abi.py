@@ -14,6 +14,8 @@ ERRORS = {
 # every symbol include/ptl.h declares
 EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_upload", "ptl_compile_resident",
            "ptl_result_device", "ptl_compile_batch", "ptl_upload_batch", "ptl_compile_resident_batch",
+           "ptl_server_create", "ptl_server_destroy", "ptl_server_submit", "ptl_server_wait",
+           "ptl_server_release", "ptl_server_stats",
            "ptl_debug_dump", "ptl_debug_stop_after",
            "ptl_debug_exclusive_scan", "ptl_profile_enable", "ptl_profile_get")
 
@@ -38,6 +40,10 @@ class PtlResult(C.Structure):
     ]
 
 
+class PtlServeTimes(C.Structure):
+    _fields_ = [("queue_ns", C.c_uint64), ("total_ns", C.c_uint64), ("batch_programs", C.c_uint32)]
+
+
 def bind(lib):
     lib.ptl_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_uint64]
     lib.ptl_create.restype = C.c_int
@@ -59,6 +65,17 @@ def bind(lib):
     lib.ptl_upload_batch.restype = C.c_int
     lib.ptl_compile_resident_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(PtlResult)]
     lib.ptl_compile_resident_batch.restype = C.c_int
+    lib.ptl_server_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_uint64, C.c_uint32, C.c_uint32]
+    lib.ptl_server_create.restype = C.c_int
+    lib.ptl_server_destroy.argtypes = [C.c_void_p]
+    lib.ptl_server_submit.argtypes = [C.c_void_p, C.POINTER(PtlAst), C.POINTER(C.c_uint64)]
+    lib.ptl_server_submit.restype = C.c_int
+    lib.ptl_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(PtlResult), C.POINTER(PtlServeTimes)]
+    lib.ptl_server_wait.restype = C.c_int
+    lib.ptl_server_release.argtypes = [C.c_void_p, C.c_uint64]
+    lib.ptl_server_release.restype = C.c_int
+    lib.ptl_server_stats.argtypes = [C.c_void_p] + [C.POINTER(C.c_uint64)] * 4
+    lib.ptl_server_stats.restype = C.c_int
     lib.ptl_debug_dump.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_uint64]
     lib.ptl_debug_dump.restype = C.c_int64
     lib.ptl_debug_stop_after.argtypes = [C.c_void_p, C.c_int]

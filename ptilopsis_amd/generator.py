@@ -146,6 +146,58 @@ class Context:
         return out[:n]
 
 
+class Server:
+    """ptl_server: the request loop over the batched pipeline (continuous
+    batching, admission by estimated HBM footprint)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_batch: int = 4096, workers: int = 2):
+        self.lib = hip_lib()
+        h = C.c_void_p()
+        s = self.lib.ptl_server_create(C.byref(h), device, hbm_budget, max_batch, workers)
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        self.h = h
+
+    def submit(self, tree: Tree) -> int:
+        keep = [np.ascontiguousarray(a) for a in tree.arrays()]
+        a = abi.PtlAst(*[k.ctypes.data for k in keep], tree.n, tree.max_depth)
+        t = C.c_uint64()
+        s = self.lib.ptl_server_submit(self.h, C.byref(a), C.byref(t))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return int(t.value)
+
+    def wait(self, ticket: int) -> tuple[dict, dict]:
+        """(result, times) of a submitted request; its storage is released."""
+        r = abi.PtlResult()
+        tm = abi.PtlServeTimes()
+        s = self.lib.ptl_server_wait(self.h, ticket, C.byref(r), C.byref(tm))
+        try:
+            if s != abi.PTL_OK:
+                raise PtlError(s)
+            out = result_dict(r)
+        finally:
+            self.lib.ptl_server_release(self.h, ticket)
+        return out, {"queue_ns": int(tm.queue_ns), "total_ns": int(tm.total_ns),
+                     "batch_programs": int(tm.batch_programs)}
+
+    def stats(self) -> dict:
+        v = [C.c_uint64() for _ in range(4)]
+        self.lib.ptl_server_stats(self.h, *[C.byref(x) for x in v])
+        return dict(zip(("batches", "requests_done", "nodes_done", "largest_batch"), (int(x.value) for x in v)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ptl_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def result_dict(r: abi.PtlResult, copy_out: bool = True) -> dict:
     def arr(p, n):
         if not copy_out or n == 0:

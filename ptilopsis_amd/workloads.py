@@ -120,3 +120,20 @@ def stress_program(seed: int, nodes: int = 64_000) -> Tree:
 
 def stress_1024(n_programs: int = 1024, seed: int = SEED_STRESS, nodes: int = 64_000) -> list[Tree]:
     return [stress_program((seed << 32) + p, nodes) for p in range(n_programs)]
+
+
+def serve_pool(n: int = 384, seed: int = SEED_REQUESTS) -> list[Tree]:
+    """Programs for the serving stream: request r draws its size from the
+    log-normal (median 4k, sigma 2.0, clipped to [200, 4M]) and its program from
+    seed SEED_REQUESTS * 2^32 + r.  A stream of more requests than the pool
+    reuses them in order (request r -> program r mod n)."""
+    sizes = request_sizes(n, seed, 4000, 2.0, 200, 4_000_000)
+    return [program_of_size((seed << 32) + r, int(s)) for r, s in enumerate(sizes)]
+
+
+def arrivals(rate: float, seconds: float, seed: int = SEED_ARRIVALS) -> np.ndarray:
+    """Poisson arrival times (s) at `rate` requests/s over `seconds`."""
+    rng = np.random.default_rng(seed + int(rate))
+    gaps = rng.exponential(1.0 / rate, int(rate * seconds * 1.2) + 16)
+    t = np.cumsum(gaps)
+    return t[t < seconds]
