@@ -23,9 +23,18 @@
 //   regalloc     k_ra_walk (functions not in lane 0 of an 8-function group:
 //                independent), k_ra_lane0 (lane-0 functions in step lockstep:
 //                they share the first 64-entry region of the reference's
-//                register_state snapshot, :1290-1354), then counts, scan, emit
-//                (:1422-1697), function table fix and frame words (:1714-1811).
+//                register_state snapshot, :1290-1354), k_ra_serial (exact
+//                in-order walk for programs where two functions name one
+//                register), then counts, scan, emit (:1422-1697), function
+//                table fix and frame words (:1714-1811).
 //   fix_jumps    size + scan + fused move/encode/postprocess (:1853-1993, 1995-2011).
+//
+// Batches: P programs concatenated node-wise run through the same kernels.
+// Each element finds its program by binary search in the program tables (node,
+// word and function bases); isn_gen computes in the program's own word indices
+// (register numbers are ORed with them) and then offsets virtual registers and
+// jump targets by the program's word base, so every program's words are
+// exactly those of its own compile, back to back.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
@@ -1066,7 +1075,8 @@ DEV bool sym_swapped(const Dev &d, uint32_t v) { return (d.sym[v] >> 8) != 0; }
 // read such a result from a parent slot, :842), and for calls with many
 // arguments the result can name a word of another function.  Such
 // names are noted here; the host then checks that no other function names the
-// same register (k_xref_check) and rejects the program otherwise.
+// same register (k_xref_check), and reallocates every program where one does
+// in the reference's exact step and group order (k_ra_serial).
 DEV void xref_note(const Dev &d, uint32_t v, uint32_t f) {
     const uint32_t e = v - 64u;
     if (e >= d.W0) return;  // out of range: flagged by the caller's bounds checks
