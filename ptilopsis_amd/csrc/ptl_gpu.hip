@@ -1545,6 +1545,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
     __shared__ uint32_t wkey[2][64];
     __shared__ uint32_t flip[64];                             // flip bits per snapshot entry
     __shared__ uint32_t mk[2][2];                             // snapshot entries marked at step i
+    __shared__ uint32_t keyed[2][2];                          // snapshot entries rewritten in the commit of step i
     __shared__ int changed;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     // this workgroup's program: groups of 8 functions from fbase, L of them
@@ -1578,6 +1579,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
     uint32_t qmin = 0xFFFFFFFFu, qmax = 0;  // wave 0: group lookups of step i+1
     if (tid < 64) {
         Sv[0][tid] = -1; So[0][tid] = -1; wkey[0][tid] = 0;
+        if (tid < 2) keyed[1][tid] = 0;
         Lmin[0][tid] = maxsize ? gmin[tid] : 0xFFFFFFFFu; Lmax[0][tid] = maxsize ? gmax[tid] : 0u;
         Omin[0][tid] = 0xFFFFFFFFu; Omax[0][tid] = 0;
         if (maxsize > 1) { qmin = gmin[64u + tid]; qmax = gmax[64u + tid]; }
@@ -1694,6 +1696,13 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         // P4: snapshot of step i+1, commit of the symbol state, marks; then the
         // fields of step i+1 into LDS and one batch of loads for the next step
         const uint64_t marked = ((uint64_t)mk[c][1] << 32) | mk[c][0];
+        // entries whose region-0 state the commit of step i-1 rewrote; every
+        // other entry a group owned at the start of step i-1 it still owns
+        const uint64_t kprev = ((uint64_t)keyed[n][1] << 32) | keyed[n][0];
+        if (tid < 64) {
+            const uint64_t kb = __ballot(wkey[c][tid] != 0);
+            if (tid == 0) { keyed[c][0] = (uint32_t)kb; keyed[c][1] = (uint32_t)(kb >> 32); }
+        }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const uint32_t t = tid + q * nth;
@@ -1718,7 +1727,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 cur_rd[j] = nx ? f.nw.x : 0u; cur_rs1[j] = nx ? f.nw.y : 0u; cur_rs2[j] = nx ? f.nw.z : 0u;
             }
             if (f.own) {
-                const uint64_t o = ra0_owned(f.own, Sv[c], So[c], j);
+                const uint64_t o = (f.own & ~kprev) | ra0_owned(f.own & kprev, Sv[c], So[c], j);
                 uint64_t m = o & marked;
                 while (m) {
                     const uint32_t reg = (uint32_t)__builtin_ctzll(m);
