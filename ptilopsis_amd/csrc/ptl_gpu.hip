@@ -1262,10 +1262,13 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
                 ((int32_t)r.rs2 > 63 && r.rs2 - 64u >= d.W0)) {
                 atomicOr(d.flags, EF_OOB);
             } else {
+                // only entries whose value changes are stored; a destination is
+                // skipped when an operand names it (the operand's store is later)
                 const uint32_t self = 64u + start + i;
-                if ((int32_t)r.ord > 63) put(r.ord, r.rdr, r.ord == self);
-                if ((int32_t)r.rs1 > 63) put(r.rs1, r.r1r, false);
-                if ((int32_t)r.rs2 > 63) put(r.rs2, r.r2r, false);
+                const bool vo = (int32_t)r.ord > 63, v1 = (int32_t)r.rs1 > 63, v2 = (int32_t)r.rs2 > 63;
+                if (vo && !(v1 && r.rs1 == r.ord) && !(v2 && r.rs2 == r.ord)) put(r.ord, r.rdr, r.ord == self);
+                if (v1 && !(v2 && r.rs2 == r.rs1) && r.r1r != raw1) put(r.rs1, r.r1r, false);
+                if (v2 && r.r2r != raw2) put(r.rs2, r.r2r, false);
             }
         }
         lds_barrier();
@@ -1296,6 +1299,155 @@ __global__ void k_walk_merge(uint32_t *gmin, uint32_t *gmax, uint64_t stride, ui
         mx = max(mx, gmax[c * stride + e]);
     }
     gmin[e] = mn; gmax[e] = mx;
+}
+
+// Programs of at most 8 functions (one group): the reference's vector loop
+// runs them as one 8-lane vector, and only its lane 0 writes region 0 of
+// register_state.  One wave holds 8 such programs, lane 8s + k walking function
+// k of program slot s (the walk's register ring, LDS cache and word staging).
+// Per step the 8 lanes' swap lookups are ORed across the group; lane 0 then
+// marks the snapshot entries looked up (not one its own current word names:
+// its reset lands after a same-group mark) and updates region 0 with its own
+// write and resets.  Every lookup and mark of the step comes after every
+// operand read of the step, as in the vector loop.  A mark on a register outside
+// lane 0's own function cannot be kept in its cache: the program then takes the
+// exact serial path.
+__global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, uint32_t np) {
+    __shared__ uint32_t cache[WALK_CACHE][64];
+    __shared__ uint4 words[WALK_CHUNK][64];
+    __shared__ uint32_t fstart[64], fsize[64];
+    __shared__ int32_t Sv[8][64];
+    const uint32_t lane = threadIdx.x, slot = lane >> 3, k = lane & 7u;
+    const uint32_t pi = blockIdx.x * 8u + slot;
+    const uint32_t prg = pi < np ? plist[pi] : 0xFFFFFFFFu;
+    const uint32_t fb = prg != 0xFFFFFFFFu ? d.pf[prg] : 0u;
+    const uint32_t nf = prg != 0xFFFFFFFFu ? d.pf[prg + 1] - fb : 0u;
+    const bool act = k < nf && nf <= 8u;
+    if (prg != 0xFFFFFFFFu && nf > 8u) atomicOr(d.flags, EF_OOB);
+    const uint32_t f = fb + k;
+    const uint32_t start = act ? d.fstarts[f] : 0u;
+    const uint32_t size = act ? d.fsizes[f] : 0u;
+    const bool con = size < (1u << 22);
+    WcRing ring{0, 0, 0, 0, 0, 0, 0, 0};
+    auto get = [&](uint32_t v) -> uint32_t {
+        const uint32_t x = v - 64u - start;
+        if (!con || x >= size) return d.sym[v - 64u];
+        const uint32_t e = wc_ring_at(ring, x & 7u);
+        if (e && (e >> 10) == x) return e & 0x1FFu;
+        return wc_lds_get(d, cache, lane, start, x);
+    };
+    auto put = [&](uint32_t v, uint32_t val, bool own) {
+        const uint32_t x = v - 64u - start;
+        if (!con || x >= size) { d.sym[v - 64u] = (uint16_t)val; return; }
+        const uint32_t e = wc_ring_at(ring, x & 7u);
+        if (e && (e >> 10) == x) { wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val); return; }
+        if (!own) { wc_lds_put(d, cache, lane, start, x, val); return; }
+        if (e) wc_lds_put(d, cache, lane, start, e >> 10, e & 0x1FFu);
+        wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val);
+    };
+    for (uint32_t q = 0; q < WALK_CACHE; ++q) cache[q][lane] = 0;
+    fstart[lane] = start; fsize[lane] = size;
+    Sv[slot][k] = -1; Sv[slot][k + 8] = -1; Sv[slot][k + 16] = -1; Sv[slot][k + 24] = -1;
+    Sv[slot][k + 32] = -1; Sv[slot][k + 40] = -1; Sv[slot][k + 48] = -1; Sv[slot][k + 56] = -1;
+    uint32_t wmax = size;
+    for (int s = 32; s > 0; s >>= 1) { uint32_t o = __shfl_xor(wmax, s, 64); wmax = wmax > o ? wmax : o; }
+    // a mark sets the swapped flag in place: in the ring, in the LDS cache, or
+    // as a byte store to HBM (the register bits stay)
+    auto mark = [&](uint32_t v) {
+        const uint32_t x = v - 64u - start;
+        const uint32_t e = wc_ring_at(ring, x & 7u);
+        if (e && (e >> 10) == x) { wc_ring_set(ring, x & 7u, e | 0x100u); return; }
+        const uint32_t ce = cache[x % WALK_CACHE][lane];
+        if (ce && (ce >> 10) == x) { cache[x % WALK_CACHE][lane] = ce | 0x100u; return; }
+        sym_mark(d, v - 64u);
+    };
+    uint64_t life = 0x1Full;
+    uint64_t valid = 0;  // lane 0 of a slot: region-0 entries holding a virtual register
+    uint64_t mkd = 0;    // lane 0: entries whose register is known to be marked already
+    __syncthreads();
+    for (uint32_t i = 0; i < wmax; ++i) {
+        if (i % WALK_CHUNK == 0) {
+            const uint32_t kk = lane % WALK_CHUNK, g0 = lane / WALK_CHUNK;
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                uint4 tmp[16];
+#pragma unroll
+                for (uint32_t r = 0; r < 16; ++r) {
+                    const uint32_t g = g0 + 2u * (16u * h + r);
+                    tmp[r] = i + kk < fsize[g] ? d.w0_f4[fstart[g] + i + kk] : make_uint4(0, 0, 0, 0);
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < 16; ++r) words[kk][g0 + 2u * (16u * h + r)] = tmp[r];
+            }
+            __syncthreads();
+        }
+        const bool on = (int32_t)i < (int32_t)size;
+        RaStep r{};
+        uint64_t look = 0;
+        uint32_t raw1 = 0, raw2 = 0;
+        if (on) {
+            const uint4 w = words[i % WALK_CHUNK][lane];
+            raw1 = virt_ok(d, w.y) ? get(w.y) : 0u;
+            raw2 = virt_ok(d, w.z) ? get(w.z) : 0u;
+            // this step's commit may clear a set swapped flag (an operand's, or a
+            // destination that is not the word's own new register): forget what
+            // is marked
+            if (((raw1 | raw2) & 0x100u) || ((int32_t)w.x > 63 && w.x != 64u + start + i)) mkd = 0;
+            r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1 & 0xFFu, raw2 & 0xFFu, (raw1 >> 8) & 1u, (raw2 >> 8) & 1u, life);
+            xref_check_fields(d, f, start, size, r.ord, r.rs1, r.rs2);
+            if (r.swd) look |= 1ull << r.rdr;
+            if (r.sw1) look |= 1ull << r.r1r;
+            if (r.sw2) look |= 1ull << r.r2r;
+            life = r.newlife;
+        }
+        // the group's lookups of this step, at every lane of the group
+        uint32_t lo = (uint32_t)look, hi = (uint32_t)(look >> 32);
+#pragma unroll
+        for (int s = 1; s < 8; s <<= 1) { lo |= (uint32_t)__shfl_xor((int)lo, s, 64); hi |= (uint32_t)__shfl_xor((int)hi, s, 64); }
+        if (k == 0) {
+            // marks on the step-start snapshot, before any commit of the step
+            uint64_t m = (((uint64_t)hi << 32) | lo) & valid & ~mkd;
+            while (m) {
+                const uint32_t reg = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const uint32_t v = (uint32_t)Sv[slot][reg];
+                if (on && (v == r.ord || v == r.rs1 || v == r.rs2)) continue;
+                if (v - 64u - start < size && con) { mark(v); mkd |= 1ull << reg; }
+                else { atomicOr(d.flags, EF_XCONFLICT); d.cflag[prg] = 1; }
+            }
+        }
+        if (on) {
+            // register_state updates (ra_commit_syms): destination, then rs1, then rs2
+            if (((int32_t)r.ord > 63 && r.ord - 64u >= d.W0) || ((int32_t)r.rs1 > 63 && r.rs1 - 64u >= d.W0) ||
+                ((int32_t)r.rs2 > 63 && r.rs2 - 64u >= d.W0)) {
+                atomicOr(d.flags, EF_OOB);
+            } else {
+                // only entries whose value changes are stored; a destination is
+                // skipped when an operand names it (the operand's store is later)
+                const uint32_t self = 64u + start + i;
+                const bool vo = (int32_t)r.ord > 63, v1 = (int32_t)r.rs1 > 63, v2 = (int32_t)r.rs2 > 63;
+                if (vo && !(v1 && r.rs1 == r.ord) && !(v2 && r.rs2 == r.ord)) put(r.ord, r.rdr, r.ord == self);
+                if (v1 && !(v2 && r.rs2 == r.rs1) && r.r1r != raw1) put(r.rs1, r.r1r, false);
+                if (v2 && r.r2r != raw2) put(r.rs2, r.r2r, false);
+            }
+            if (k == 0) {  // region 0: the write, then the resets (a reset of the written register is skipped)
+                if (r.ord) {
+                    Sv[slot][r.rdr] = (int32_t)r.ord;
+                    valid = (int32_t)r.ord >= 64 ? valid | (1ull << r.rdr) : valid & ~(1ull << r.rdr);
+                    mkd &= ~(1ull << r.rdr);
+                }
+                if (r.r1r != r.rdr) { Sv[slot][r.r1r] = -1; valid &= ~(1ull << r.r1r); }
+                if (r.r2r != r.rdr) { Sv[slot][r.r2r] = -1; valid &= ~(1ull << r.r2r); }
+            }
+        }
+    }
+    if (con) {
+        for (uint32_t q = 0; q < 8; ++q) {
+            const uint32_t e = wc_ring_at(ring, q);
+            if (e) wc_lds_put(d, cache, lane, start, e >> 10, e & 0x1FFu);
+        }
+        for (uint32_t q = 0; q < WALK_CACHE; ++q) wc_evict(d, start, cache[q][lane]);
+    }
 }
 
 // Lane-0 functions in step lockstep, one workgroup.  Region 0 of the
