@@ -14,10 +14,10 @@ large_16m      one program, ~16M nodes, 20k functions.  Per-function sizes are
                counts are kept, the range shrinks to ~[21, 4.3k]).
 batch_4096     4096 programs, log-normal sizes (median 8k, sigma 1.5) clipped to
                [1k, 1M], ~100M nodes in total.
-stress_1024    1024 programs of ~64k nodes: 6-16 mixed args, wide balanced
-               expressions of depth 10-14, chains of if/while nested 8-16 deep,
-               ~30 % floats.  Functions (of ~1.5k-node targets) are added until
-               the program reaches 64k nodes.
+stress_1024    1024 programs of ~64k nodes (each within [48k, 80k]): 6-16 mixed
+               args, wide balanced expressions of depth 10-14, chains of if/while
+               nested 8-16 deep, ~30 % floats.  Functions (of ~1.5k-node targets)
+               are added until the program reaches ~64k nodes.
 serve_stream   request sizes log-normal (median 4k, sigma 2.0) clipped to [200, 4M].
 """
 from __future__ import annotations
@@ -110,12 +110,17 @@ def batch_4096(n_programs: int = 4096, seed: int = SEED_BATCH) -> list[Tree]:
 
 
 def stress_program(seed: int, nodes: int = 64_000) -> Tree:
-    """Functions of ~1.5k-node targets until the program holds 84 % of `nodes`
-    nodes; the last function's overshoot brings the mean to about `nodes`."""
+    """Functions of ~1.5k-node targets until the program holds 90 % of `nodes`
+    nodes; the last function's overshoot brings the mean to about `nodes`.  A
+    draw outside [0.75, 1.25] x `nodes` (one deep chain of wide expressions can
+    add ~100k nodes) is regenerated with a derived seed."""
     cap = nodes // 64 + 16
-    src, _ = program_source(seed, cap, np.full(cap, 1500, np.uint32), STRESS_NESTED,
-                            total_target=int(nodes * 0.84))
-    return parse_source(src)
+    for attempt in range(64):
+        src, n = program_source(seed + (attempt << 40), cap, np.full(cap, 1500, np.uint32), STRESS_NESTED,
+                                total_target=int(nodes * 0.9))
+        if 0.75 * nodes <= n <= 1.25 * nodes:
+            return parse_source(src)
+    raise RuntimeError("could not hit the stress program's node window")
 
 
 def stress_1024(n_programs: int = 1024, seed: int = SEED_STRESS, nodes: int = 64_000) -> list[Tree]:
