@@ -54,7 +54,8 @@ typedef struct {
     uint64_t n_words_gen;         /* W0: words after isn_gen */
     uint64_t n_words_alloc;       /* W1: words after regalloc */
     uint32_t opt_iterations;      /* optimize fixpoint launches */
-    uint32_t tie_slots;           /* word slots written by two nodes of equal depth */
+    uint32_t tie_slots;           /* word slots written by two nodes of equal depth; 0xFFFFFFFF: not
+                                     counted (inputs of 4M nodes or more always take the depth sort) */
 } ptl_result;
 
 typedef struct ptl_ctx ptl_ctx;

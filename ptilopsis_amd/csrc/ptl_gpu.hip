@@ -106,10 +106,12 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block) {
 // ------------------------------------------------------- reduce-then-scan
 // Exclusive u32 scan over n elements in three launches with no inter-block
 // waiting: per-tile sums, one workgroup scanning the tile sums, then each tile
-// rescans its elements with its offset.  A tile is 256 threads x 16 items; each
-// thread owns 16 consecutive elements, a wave64 shuffle scan and LDS combine
-// them.  Op::value(i) produces element i, Op::emit(i, exclusive, value)
-// consumes the result.
+// rescans its elements with its offset.  A tile is 256 threads x 16 items.
+// Elements are produced striped (element k*256 + t by thread t, so every load
+// of Op::value is coalesced) and, for the rescan, transposed through LDS so
+// that each thread owns 16 consecutive elements, which a wave64 shuffle scan
+// and LDS combine.  Op::value(i) produces element i, Op::emit(i, exclusive,
+// value) consumes the result.
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr uint64_t SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
@@ -132,11 +134,11 @@ DEV uint32_t wave_sum(uint32_t x) {
 template <class Op>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(Op op, uint64_t n, uint32_t *tile_sums) {
     __shared__ uint32_t s_w[SCAN_BLOCK / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        uint64_t i = base + k;
+        uint64_t i = base + (uint64_t)k * SCAN_BLOCK;
         if (i < n) sum += op.value(i);
     }
     sum = wave_sum(sum);
@@ -176,13 +178,20 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t *sums, uint32_t nt
 template <class Op>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(Op op, uint64_t n, const uint32_t *tile_pre) {
     __shared__ uint32_t s_w[SCAN_BLOCK / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 16];  // element e at e + e/16 (one pad word per 16)
+    const uint64_t tb = (uint64_t)blockIdx.x * SCAN_TILE;
+    const uint64_t base = tb + (uint64_t)threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const uint32_t e = (uint32_t)k * SCAN_BLOCK + threadIdx.x;
+        s_v[e + e / 16u] = tb + e < n ? op.value(tb + e) : 0u;
+    }
+    __syncthreads();
     uint32_t v[SCAN_ITEMS];
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        uint64_t i = base + k;
-        v[k] = i < n ? op.value(i) : 0u;
+        v[k] = s_v[threadIdx.x * (SCAN_ITEMS + 1) + k];  // element t*16 + k
         sum += v[k];
     }
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
