@@ -107,7 +107,10 @@ int ptl_compile_resident_batch(ptl_ctx *ctx, int slot, int copy_out, ptl_result 
  * requests arriving while batches run join the next batch and up to `workers`
  * batches are on the device at once.  A request's device footprint is
  * estimated as 1 MiB + 384 B per node; one larger than the whole budget is
- * refused (PTL_ERR_ADMISSION).  hbm_budget_bytes 0 means 64 GiB.
+ * refused (PTL_ERR_ADMISSION).  hbm_budget_bytes 0 means 64 GiB.  Requests of
+ * 256k nodes or more queue apart from smaller ones (even workers take small
+ * requests first, odd workers large ones), so small requests do not wait
+ * behind a large one's batch.
  * ptl_server_submit copies the arrays and returns at once; ptl_server_wait
  * blocks until the request is done and fills out (valid until
  * ptl_server_release of that ticket) and, if given, its times.  Each result is
