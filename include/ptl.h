@@ -144,6 +144,11 @@ int64_t ptl_debug_dump(ptl_ctx *ctx, const char *name, uint32_t *dst, uint64_t c
  * ptl_compile (0 = all stages); for stage-boundary parity tests. */
 int ptl_debug_stop_after(ptl_ctx *ctx, int stage);
 
+/* Register allocation of a compile whose only multi-group program goes through
+ * the walk and the lockstep runs the two kernels at the same time (on = 1, the
+ * default) or one after the other (on = 0); the output is the same. */
+int ptl_debug_overlap(ptl_ctx *ctx, int on);
+
 /* Per-kernel timing: with profiling on, every kernel launch of the following
  * calls is bracketed by HIP events on the context's stream.  ptl_profile_get
  * returns, for entry idx (0-based, by first launch), the kernel name, the

@@ -62,7 +62,8 @@ DEV uint32_t size_of(uint32_t ty, uint32_t rt) { return (c_bits[spec_i(ty, 0, rt
 // error flags raised by kernels (bit set in ctx->d_flags)
 // EF_XNAMED: some function names a virtual register of a word outside itself;
 // EF_XCONFLICT: such a register is also named by another function (see xref_note).
-enum : uint32_t { EF_OOB = 1, EF_CALLPATH = 4, EF_XNAMED = 16, EF_XCONFLICT = 32 };
+// EF_WAIT: the allocation lockstep gave up waiting for the concurrent walk's rows.
+enum : uint32_t { EF_OOB = 1, EF_CALLPATH = 4, EF_XNAMED = 16, EF_XCONFLICT = 32, EF_WAIT = 64 };
 
 // ------------------------------------------------------------------ helpers
 // Workgroup barrier that orders LDS only: global loads and stores stay in
@@ -1133,6 +1134,16 @@ constexpr uint32_t WALK_CHUNK = 32;
 // k_walk_merge folds: one table shared by every workgroup serializes its
 // atomics on the same few addresses.
 constexpr uint32_t WALK_COPIES = 16;
+// Walk and lockstep may run at the same time (one program): each walk
+// workgroup then publishes its table rows in chunks of WALK_PUB rows, adding 1
+// to done[chunk] (release, agent scope) once it has posted every row of the
+// chunk it takes part in.  The lockstep reads a row after done[chunk] reaches
+// need[chunk], the number of walk workgroups still running at the chunk's
+// first row.  Walk workgroups never wait, so the lockstep's wait always ends;
+// it gives up (EF_WAIT) after RA0_WAIT_TICKS of the 100 MHz wall clock.
+constexpr uint32_t WALK_PUB = 8;
+constexpr uint32_t RA0_MAXCOPIES = 8;  // table copies the lockstep folds itself
+constexpr uint64_t RA0_WAIT_TICKS = 400000000ull;  // 4 s
 
 // One program's allocation: functions from fbase (nf of them) in groups of 8
 // (L groups), maxsize = its longest function; its slices of the lane-0 group
@@ -1183,7 +1194,7 @@ DEV void wc_lds_put(const Dev &d, uint32_t (*c)[64], uint32_t lane, uint32_t sta
 
 __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint32_t *gmax_all, uint64_t stride,
                                                 uint32_t copies, const Ra0Prog *progs, const uint32_t *permw,
-                                                uint32_t nw) {
+                                                uint32_t nw, uint32_t *done) {
     __shared__ uint32_t smin[64], smax[64];
     __shared__ uint32_t cache[WALK_CACHE][64];
     __shared__ uint4 words[WALK_CHUNK][64];
@@ -1206,6 +1217,7 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
     const uint32_t tlast = one ? progs[pmin].rows - 1u : 0u;
     const uint64_t mbase = act ? progs[prg].gbase : 0u;  // own program's table and its last row
     const uint32_t mlast = act ? progs[prg].rows - 1u : 0u;
+    if (done && !one && lane == 0) atomicOr(d.flags, EF_OOB);  // publishing needs one program per workgroup
     const bool con = size < (1u << 22);  // cache on (tags hold 22 bits of word offset)
     WcRing ring{0, 0, 0, 0, 0, 0, 0, 0};
     // state of virtual register v (inside [64, 64 + W0)); then its update
@@ -1288,6 +1300,16 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
             smin[lane] = 0xFFFFFFFFu; smax[lane] = 0;
         }
         lds_barrier();
+        if (done && one) {
+            // the workgroup leaves a chunk of rows (steps past the last row all
+            // post to it): publish it after every lane's posts are visible
+            const uint32_t r = min(i, tlast), rn = min(i + 1u, tlast);
+            if (i + 1u == wmax || r / WALK_PUB != rn / WALK_PUB) {
+                __threadfence();
+                __syncthreads();
+                if (lane == 0) __hip_atomic_fetch_add(&done[r / WALK_PUB], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
     }
     if (con) {
         for (uint32_t q = 0; q < 8; ++q) {
@@ -1695,9 +1717,38 @@ DEV void ra0_keys(uint32_t *wkey, const RaStep &r, uint32_t j) {
     if (r.r2r != r.rdr) atomicMax(&wkey[r.r2r], ((j + 1u) << 2) | 3u);
 }
 
+// Lookups of table row r by the walked functions (entry `tid`): min and max
+// over the table's `copies` copies, `stride` apart.  Every load is issued
+// before the first is used.
+DEV void ra0_row(const uint32_t *gmin, const uint32_t *gmax, uint64_t stride, uint32_t copies, uint32_t r,
+                 uint32_t tid, uint32_t &mn, uint32_t &mx) {
+    const uint64_t e = (uint64_t)r * 64u + tid;
+    uint32_t a[RA0_MAXCOPIES], b[RA0_MAXCOPIES];
+#pragma unroll
+    for (uint32_t c = 0; c < RA0_MAXCOPIES; ++c) {
+        const uint64_t o = (c < copies ? c : 0u) * stride + e;
+        a[c] = gmin[o]; b[c] = gmax[o];
+    }
+    mn = a[0]; mx = b[0];
+#pragma unroll
+    for (uint32_t c = 1; c < RA0_MAXCOPIES; ++c) { mn = min(mn, a[c]); mx = max(mx, b[c]); }
+}
+
+// Waits (every lane of the wave) until the walk has published row chunk c.
+DEV void ra0_wait(const Dev &d, const uint32_t *done, const uint32_t *need, uint32_t c) {
+    const uint32_t want = need[c];
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(&done[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+        if (wall_clock64() - t0 > RA0_WAIT_TICKS) { atomicOr(d.flags, EF_WAIT); return; }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
 template <int PER, uint32_t NL>
 __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, const uint32_t *gmin_all,
-                                                   const uint32_t *gmax_all, const uint32_t *perm_all) {
+                                                   const uint32_t *gmax_all, const uint32_t *perm_all,
+                                                   const uint32_t *done, const uint32_t *need, uint32_t copies,
+                                                   uint64_t stride) {
     __shared__ uint32_t cur_rd[NL], cur_rs1[NL], cur_rs2[NL];
     __shared__ uint32_t rrs[NL];                              // step result per group: r1r | r2r << 8 | rdr << 16
     __shared__ int32_t Sv[2][64], So[2][64];                  // region-0 snapshot of step i / i+1
@@ -1741,9 +1792,17 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
     if (tid < 64) {
         Sv[0][tid] = -1; So[0][tid] = -1; wkey[0][tid] = 0;
         if (tid < 2) keyed[1][tid] = 0;
-        Lmin[0][tid] = maxsize ? gmin[tid] : 0xFFFFFFFFu; Lmax[0][tid] = maxsize ? gmax[tid] : 0u;
+        uint32_t m0 = 0xFFFFFFFFu, x0 = 0u;
+        if (maxsize) {
+            if (done) ra0_wait(d, done, need, 0);
+            ra0_row(gmin, gmax, stride, copies, 0, tid, m0, x0);
+        }
+        Lmin[0][tid] = m0; Lmax[0][tid] = x0;
         Omin[0][tid] = 0xFFFFFFFFu; Omax[0][tid] = 0;
-        if (maxsize > 1) { qmin = gmin[64u + tid]; qmax = gmax[64u + tid]; }
+        if (maxsize > 1) {
+            if (done && 1u % WALK_PUB == 0) ra0_wait(d, done, need, 1u / WALK_PUB);
+            ra0_row(gmin, gmax, stride, copies, 1, tid, qmin, qmax);
+        }
     }
     __syncthreads();
     uint32_t fb = 0;   // wave 0: flip bits of snapshot register tid
@@ -1916,7 +1975,10 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         }
         if (tid < 64) {
             qmin = 0xFFFFFFFFu; qmax = 0;
-            if (i + 2 < maxsize) { qmin = gmin[(uint64_t)(i + 2) * 64u + tid]; qmax = gmax[(uint64_t)(i + 2) * 64u + tid]; }
+            if (i + 2 < maxsize) {
+                if (done && (i + 2) % WALK_PUB == 0) ra0_wait(d, done, need, (i + 2) / WALK_PUB);
+                ra0_row(gmin, gmax, stride, copies, i + 2, tid, qmin, qmax);
+            }
         }
 #pragma unroll
         for (int q = 0; q < PER; ++q)

@@ -137,6 +137,30 @@ def test_gpu_one_million_nodes(gpu_ctx):
     assert_same(gpu_ctx.compile(t), oracle.compile_tree(t))
 
 
+@pytest.mark.parametrize("which", ["single_50k", "300k"])
+def test_gpu_walk_and_lockstep_overlap(gpu_ctx, which):
+    # one program: the walk publishes its lookup rows while the lockstep runs;
+    # run one after the other, the output is the same, and both equal the oracle
+    t = W.single_50k() if which == "single_50k" else W.program_of_size(0x5EED0008, 300_000)
+    want = oracle.compile_tree(t)
+    got = {}
+    try:
+        for on in (False, True):
+            say("walk/lockstep overlap", on, which, t.n)
+            gpu_ctx.set_overlap(on)
+            gpu_ctx.profile(True)
+            got[on] = gpu_ctx.compile(t)
+            names = gpu_ctx.kernel_times()
+            gpu_ctx.profile(False)
+            assert "k_ra_walk" in names and "k_ra_lane0" in names
+            assert ("k_walk_merge" in names) == (not on)
+    finally:
+        gpu_ctx.set_overlap(True)
+        gpu_ctx.profile(False)
+    assert_same(got[False], want)
+    assert_same(got[True], want)
+
+
 def test_gpu_many_small_functions(gpu_ctx):
     src, _ = W.program_source(31, 300, min_stmts=1, max_stmts=3)
     t = parse_source(src)
