@@ -15,10 +15,17 @@ The JSON line also carries:
                 generator) timed on the same input on this host, whose output
                 also checks the GPU result word for word (bit_exact);
   stages_ms     per-stage device time;
+  stages_roofline  each stage's algorithmic bytes (stage-boundary elements read
+                and written once) over its device time, as a fraction of the
+                8 TB/s HBM peak, and the measured HBM bytes of the committed
+                rocprofv3 PMC summary over the same time where there is one;
   batched       the same metric for the batched pipeline on batch_4096 (4096
                 programs, ~100M nodes, one pass) and stress_1024 (1024
                 register/jump stress programs of ~64k nodes), each with
-                sampled programs checked word for word against the oracle;
+                sampled programs checked word for word against the oracle, and
+                for batch_4096 a multi-core host baseline (the oracle on a pool
+                of host threads, one program per task, over the first ~20M
+                nodes of the batch, checked against the GPU's output);
   serving       the request loop (ptl_server) under Poisson arrivals at each
                 rate: p50/p99 submit-to-result latency, nodes/s, batches, and
                 whether every result equals the program compiled alone.
@@ -73,6 +80,91 @@ def kernel_alg_bytes(name: str, n: int, w0: int, w1: int, w2: int, f: int) -> fl
     return table.get(name)
 
 
+STAGES = ("preprocess", "isn_cnt", "isn_gen", "optimize", "regalloc", "fix_jumps", "postprocess")
+
+
+def stage_alg_bytes(n: int, w0: int, w1: int, w2: int, f: int) -> dict:
+    """Stage-boundary compulsory bytes of each stage: every stage input element
+    read once and every output element written once (u8 types, 4 B else)."""
+    return {"preprocess": 20 * n, "isn_cnt": 18 * n + 16 * f, "isn_gen": 22 * n + 8 * f + 20 * w0,
+            "optimize": 20 * w0, "regalloc": 24 * w0 + 20 * w1 + 24 * f,
+            "fix_jumps": 20 * w1 + 20 * w2 + 24 * f, "postprocess": 20 * w2}
+
+
+def measured_stage_traffic(workload: str) -> dict:
+    """HBM bytes per compile of each stage from the committed rocprofv3 PMC
+    summary of `workload` (its per-stage table), if any."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.md")
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 3 and cells[0].startswith("stage "):
+            try:
+                out[cells[0][6:]] = float(cells[2].replace(",", ""))
+            except ValueError:
+                pass
+    return out
+
+
+def stage_roofline(stages_ms: dict, n: int, w0: int, w1: int, w2: int, f: int, workload: str) -> dict:
+    """Each stage as a fraction of the HBM peak: algorithmic bytes over its
+    device time, and (where profiled) the measured HBM bytes over the same time.
+    fix_jumps and postprocess run as one fused kernel: they are reported together."""
+    alg = stage_alg_bytes(n, w0, w1, w2, f)
+    meas = measured_stage_traffic(workload)
+    out = {}
+    for st in ("preprocess", "isn_cnt", "isn_gen", "optimize", "regalloc", "fix_jumps+postprocess"):
+        parts = st.split("+")
+        ms = sum(stages_ms[p] for p in parts)
+        a = sum(alg[p] for p in parts)
+        rec = {"ms": ms, "alg_bytes": a, "alg_GBps": a / (ms * 1e-3) / 1e9,
+               "alg_frac": a / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if st in meas:
+            rec["traffic_bytes"] = meas[st]
+            rec["traffic_GBps"] = meas[st] / (ms * 1e-3) / 1e9
+            rec["traffic_frac"] = rec["traffic_GBps"] / HBM_PEAK_GBS
+        out[st] = rec
+    return out
+
+
+def host_cpu() -> dict:
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
+def cpu_batch_baseline(trees, res, max_nodes: int) -> dict:
+    """The oracle on a pool of host threads, one program per task (programs are
+    independent; ctypes releases the GIL), over the first programs of a batch up
+    to max_nodes nodes, each checked word for word against the GPU's result."""
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle
+    oracle.lib()
+    k, tot = 0, 0
+    while k < len(trees) and tot < max_nodes:
+        tot += trees[k].n
+        k += 1
+    threads = max(1, min((os.cpu_count() or 2) - 1, 15))
+    keys = ("words", "func_ids", "func_starts", "func_ends", "func_sizes")
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        outs = list(ex.map(oracle.compile_tree, trees[:k]))
+    el = time.perf_counter() - t0
+    same = all(all(np.array_equal(res[p][q], outs[p][q]) for q in keys) for p in range(k))
+    return {"value": tot / el, "unit": "nodes/s", "cores": threads, "kind": "port",
+            "sample": f"first {k} programs of the batch ({tot} nodes), the oracle (C restatement of "
+                      f"rv_generator_avx at one thread, AVX2 build) on {threads} host threads, one program per "
+                      f"task, {el:.1f}s", "bit_exact": bool(same)}
+
+
 def measured_traffic(kernel: str, workload: str) -> float | None:
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     of `workload` (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
@@ -89,7 +181,7 @@ def measured_traffic(kernel: str, workload: str) -> float | None:
     return None
 
 
-def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: int) -> dict:
+def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: int, cpu_nodes: int) -> dict:
     """Batched pipeline over one named batch workload, resident in HBM."""
     import oracle
     from ptilopsis_amd import workloads as W
@@ -121,12 +213,18 @@ def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: i
         ok &= all(np.array_equal(res[p][k], want[k]) for k in ("words", "func_ids", "func_starts", "func_ends",
                                                                   "func_sizes"))
     log(f"{name}: {len(pick)} sampled programs bit-exact: {ok}")
-    stages = {k: v / 1e6 for k, v in zip(("preprocess", "isn_cnt", "isn_gen", "optimize", "regalloc", "fix_jumps",
-                                          "postprocess"), res[0]["stage_ns"])}
+    stages = {k: v / 1e6 for k, v in zip(STAGES, res[0]["stage_ns"])}
+    sroof = stage_roofline(stages, nodes, sum(r["W0"] for r in res), sum(r["W1"] for r in res),
+                           sum(r["n_words"] for r in res), sum(r["n_funcs"] for r in res), name)
     out = {"workload": name, "value": value, "unit": "nodes/s", "ms_per_step": el * 1e3 / steps, "steps": steps,
            "programs": len(trees), "nodes": nodes, "words": int(sum(r["n_words"] for r in res)),
-           "sampled_bit_exact": ok, "sampled_programs": pick, "stages_ms": stages,
+           "sampled_bit_exact": ok, "sampled_programs": pick, "stages_ms": stages, "stages_roofline": sroof,
            "top_kernels_ms": {k: v[0] / 1e6 for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:6]}}
+    if cpu_nodes:
+        out["cpu_baseline"] = cpu_batch_baseline(trees, res, cpu_nodes)
+        log(f"{name}: host baseline {out['cpu_baseline']['value'] / 1e6:.2f} M nodes/s on "
+            f"{out['cpu_baseline']['cores']} threads ({out['cpu_baseline']['sample']}), "
+            f"bit-exact: {out['cpu_baseline']['bit_exact']}")
     del trees, res
     return out
 
@@ -210,6 +308,9 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--overlap", type=int, default=1, choices=(0, 1),
+                    help="one-program allocation: walk and lockstep at the same time (1) or one after the other "
+                         "(0; for counter collection, which serializes kernels)")
     ap.add_argument("--serve-rates", default="500,2000,5000,20000")
     ap.add_argument("--serve-seconds", type=float, default=2.5)
     ap.add_argument("--serve-pool", type=int, default=384)
@@ -239,6 +340,7 @@ def main() -> int:
     log(f"workload {args.workload}: {tree.n} nodes, {tree.n_functions} functions, depth {tree.max_depth} "
         f"(generated+parsed in {time.time() - t0:.1f}s)")
     ctx = Context(0)
+    ctx.set_overlap(args.overlap)
     ctx.upload(0, tree)
     for i in range(args.warmup):
         r = ctx.compile_resident(0, copy_out=False)
@@ -256,8 +358,10 @@ def main() -> int:
     res = ctx.compile_resident(0, copy_out=True)
     ktimes = ctx.kernel_times()
     ctx.profile(False)
-    stages = {k: v / 1e6 for k, v in zip(("preprocess", "isn_cnt", "isn_gen", "optimize", "regalloc",
-                                          "fix_jumps", "postprocess"), res["stage_ns"])}
+    stages = {k: v / 1e6 for k, v in zip(STAGES, res["stage_ns"])}
+    sroof = stage_roofline(stages, tree.n, res["W0"], res["W1"], res["n_words"], res["n_funcs"], args.workload)
+    log("stages (ms, fraction of HBM peak by algorithmic bytes): " + ", ".join(
+        f"{k}={v['ms']:.2f}/{v['alg_frac']:.4f}" for k, v in sroof.items()))
     dom = max(ktimes.items(), key=lambda kv: kv[1][0])
     dname, (dns, dlaunch) = dom
     alg = kernel_alg_bytes(dname, tree.n, res["W0"], res["W1"], res["n_words"], res["n_funcs"])
@@ -285,14 +389,15 @@ def main() -> int:
         c1 = time.perf_counter()
         cpu = {"value": tree.n / (c1 - c0), "unit": "nodes/s", "cores": 1, "kind": "port",
                "sample": f"full {args.workload} input ({tree.n} nodes), one run of the oracle (C restatement of "
-                         f"rv_generator_avx at one thread), {c1 - c0:.1f}s"}
+                         f"rv_generator_avx at one thread, AVX2 build), {c1 - c0:.1f}s", "host": host_cpu()}
         bit_exact = all(np.array_equal(res[k], want[k]) for k in
                         ("words", "func_ids", "func_starts", "func_ends", "func_sizes"))
         log(f"cpu oracle: {c1 - c0:.1f}s; bit-exact: {bit_exact}")
 
     batched = None
     if not args.no_batch:
-        batched = [measure_batch(ctx, nm, 1 + k, args.batch_steps, 1, 8)
+        batched = [measure_batch(ctx, nm, 1 + k, args.batch_steps, 1, 8,
+                                 0 if args.no_cpu_baseline or nm != "batch_4096" else 20_000_000)
                    for k, nm in enumerate(("batch_4096", "stress_1024"))]
 
     serving = None
@@ -308,6 +413,7 @@ def main() -> int:
                    "max_depth": tree.max_depth, "words": res["n_words"], "path": "plain one-program GPU pipeline",
                    "inputs": "resident in HBM"},
         "roofline": roof, "cpu_baseline": cpu, "bit_exact": bit_exact, "stages_ms": stages,
+        "stages_roofline": sroof,
         "batched": batched, "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -146,7 +146,11 @@ int ptl_debug_stop_after(ptl_ctx *ctx, int stage);
 
 /* Register allocation of a compile whose only multi-group program goes through
  * the walk and the lockstep runs the two kernels at the same time (on = 1, the
- * default) or one after the other (on = 0); the output is the same. */
+ * default) or one after the other (on = 0); the output is the same.  on = 2
+ * runs them at the same time but has the lockstep wait for one more walk
+ * workgroup than there is: its bounded wait gives up (0.25 s) and the library
+ * redoes the allocation one kernel after the other, as it does when the device
+ * serializes the two kernels.  PTL_ERR_ARG for other values. */
 int ptl_debug_overlap(ptl_ctx *ctx, int on);
 
 /* Per-kernel timing: with profiling on, every kernel launch of the following

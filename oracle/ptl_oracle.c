@@ -44,7 +44,9 @@ enum {
 typedef struct { uint32_t *p; int64_t n; int64_t cap; } vb; /* avx_buffer<u32> */
 typedef struct { uint64_t *p; int64_t n; int64_t cap; } vb64;
 
-static uint32_t g_flags;
+/* per thread: independent oracle runs may share the process (the host baseline
+ * compiles many programs on a thread pool) */
+static _Thread_local uint32_t g_flags;
 
 static vb vb_new(int64_t n) {
     vb b; b.n = n; b.cap = (n + 15) & ~(int64_t)15;
@@ -114,7 +116,7 @@ static void excl_scan_vec(const vb *in, vb *out) {
  * median of (first+1, mid, last-1) moved to first, unguarded Hoare partition,
  * segments <= 16 left for a final insertion sort, heapsort when the limit runs
  * out.  The comparator is depth[a] < depth[b] (rv_generator_avx.cpp:539-541). */
-static const int32_t *g_key;
+static _Thread_local const int32_t *g_key;
 #define LESS(a, b) (g_key[(a)] < g_key[(b)])
 
 static void swap_u32(uint32_t *a, uint32_t *b) { uint32_t t = *a; *a = *b; *b = t; }

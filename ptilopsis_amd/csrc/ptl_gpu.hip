@@ -1139,11 +1139,15 @@ constexpr uint32_t WALK_COPIES = 16;
 // to done[chunk] (release, agent scope) once it has posted every row of the
 // chunk it takes part in.  The lockstep reads a row after done[chunk] reaches
 // need[chunk], the number of walk workgroups still running at the chunk's
-// first row.  Walk workgroups never wait, so the lockstep's wait always ends;
-// it gives up (EF_WAIT) after RA0_WAIT_TICKS of the 100 MHz wall clock.
+// first row.  Walk workgroups never wait, so when both kernels are resident
+// the lockstep's wait always ends.  Where dispatches are serialized (a profiler
+// collecting counters), the walk cannot start before the lockstep ends: a wait
+// that sees no progress for RA0_WAIT_TICKS of the 100 MHz wall clock gives up
+// for the rest of the kernel (EF_WAIT), and the host redoes the allocation with
+// the two kernels one after the other.
 constexpr uint32_t WALK_PUB = 8;
 constexpr uint32_t RA0_MAXCOPIES = 8;  // table copies the lockstep folds itself
-constexpr uint64_t RA0_WAIT_TICKS = 400000000ull;  // 4 s
+constexpr uint64_t RA0_WAIT_TICKS = 25000000ull;  // 0.25 s
 
 // One program's allocation: functions from fbase (nf of them) in groups of 8
 // (L groups), maxsize = its longest function; its slices of the lane-0 group
@@ -1734,12 +1738,14 @@ DEV void ra0_row(const uint32_t *gmin, const uint32_t *gmax, uint64_t stride, ui
     for (uint32_t c = 1; c < RA0_MAXCOPIES; ++c) { mn = min(mn, a[c]); mx = max(mx, b[c]); }
 }
 
-// Waits (every lane of the wave) until the walk has published row chunk c.
-DEV void ra0_wait(const Dev &d, const uint32_t *done, const uint32_t *need, uint32_t c) {
+// Waits (every lane of the wave) until the walk has published row chunk c;
+// `ok` turns false, for good, when a wait sees no progress for RA0_WAIT_TICKS.
+DEV void ra0_wait(const Dev &d, const uint32_t *done, const uint32_t *need, uint32_t c, bool &ok) {
+    if (!ok) return;
     const uint32_t want = need[c];
     const uint64_t t0 = wall_clock64();
     while (__hip_atomic_load(&done[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
-        if (wall_clock64() - t0 > RA0_WAIT_TICKS) { atomicOr(d.flags, EF_WAIT); return; }
+        if (wall_clock64() - t0 > RA0_WAIT_TICKS) { ok = false; atomicOr(d.flags, EF_WAIT); return; }
         __builtin_amdgcn_s_sleep(2);
     }
 }
@@ -1789,18 +1795,19 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         }
     }
     uint32_t qmin = 0xFFFFFFFFu, qmax = 0;  // wave 0: group lookups of step i+1
+    bool wok = true;                         // wave 0: the walk's rows still arrive
     if (tid < 64) {
         Sv[0][tid] = -1; So[0][tid] = -1; wkey[0][tid] = 0;
         if (tid < 2) keyed[1][tid] = 0;
         uint32_t m0 = 0xFFFFFFFFu, x0 = 0u;
         if (maxsize) {
-            if (done) ra0_wait(d, done, need, 0);
+            if (done) ra0_wait(d, done, need, 0, wok);
             ra0_row(gmin, gmax, stride, copies, 0, tid, m0, x0);
         }
         Lmin[0][tid] = m0; Lmax[0][tid] = x0;
         Omin[0][tid] = 0xFFFFFFFFu; Omax[0][tid] = 0;
         if (maxsize > 1) {
-            if (done && 1u % WALK_PUB == 0) ra0_wait(d, done, need, 1u / WALK_PUB);
+            if (done && 1u % WALK_PUB == 0) ra0_wait(d, done, need, 1u / WALK_PUB, wok);
             ra0_row(gmin, gmax, stride, copies, 1, tid, qmin, qmax);
         }
     }
@@ -1976,7 +1983,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         if (tid < 64) {
             qmin = 0xFFFFFFFFu; qmax = 0;
             if (i + 2 < maxsize) {
-                if (done && (i + 2) % WALK_PUB == 0) ra0_wait(d, done, need, (i + 2) / WALK_PUB);
+                if (done && (i + 2) % WALK_PUB == 0) ra0_wait(d, done, need, (i + 2) / WALK_PUB, wok);
                 ra0_row(gmin, gmax, stride, copies, i + 2, tid, qmin, qmax);
             }
         }

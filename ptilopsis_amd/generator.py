@@ -129,9 +129,11 @@ class Context:
     def stop_after(self, stage: int) -> None:
         self.lib.ptl_debug_stop_after(self.h, stage)
 
-    def set_overlap(self, on: bool) -> None:
-        """Walk and lockstep at the same time (default) or one after the other."""
-        self.lib.ptl_debug_overlap(self.h, 1 if on else 0)
+    def set_overlap(self, on) -> None:
+        """Walk and lockstep at the same time (True, default) or one after the
+        other (False); 2 forces the lockstep's bounded wait to give up and the
+        allocation to be redone one kernel after the other."""
+        self.lib.ptl_debug_overlap(self.h, int(on))
 
     def exclusive_scan(self, x: np.ndarray) -> np.ndarray:
         a = np.ascontiguousarray(x, dtype=np.uint32)
