@@ -26,6 +26,9 @@ The JSON line also carries:
                 for batch_4096 a multi-core host baseline (the oracle on a pool
                 of host threads, one program per task, over the first ~20M
                 nodes of the batch, checked against the GPU's output);
+  to_asm        the text output (rv_generator::to_asm) of the compiled words,
+                formatted on the device, against the HBM peak and beside the
+                oracle's restatement on a sample;
   serving       the request loop (ptl_server) under Poisson arrivals at each
                 rate: p50/p99 submit-to-result latency, nodes/s, batches, and
                 whether every result equals the program compiled alone.
@@ -179,6 +182,42 @@ def measured_traffic(kernel: str, workload: str) -> float | None:
             except ValueError:
                 return None
     return None
+
+
+def measure_asm(ctx, res: dict, with_cpu: bool) -> dict:
+    """Text output (rv_generator::to_asm) of the last compile's words, formatted
+    on the device: the length scan and the write kernel's HIP-event times, their
+    algorithmic bytes (words read once, text written once) against 8 TB/s, the
+    call's wall time (text copied to host memory included), and the oracle's
+    restatement on a sample of the same words."""
+    import oracle
+    ctx.profile(True)
+    ctx.result_asm(0)  # warm
+    t0 = time.perf_counter()
+    text = ctx.result_asm(0)
+    wall = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    ctx.profile(False)
+    words = res["n_words"]
+    dev_ns = sum(v[0] for k, v in kt.items() if k.startswith("k_asm"))
+    alg = 4 * words + len(text)
+    out = {"words": words, "text_bytes": len(text), "device_ms": dev_ns / 1e6, "wall_ms": wall * 1e3,
+           "kernels_ms": {k: v[0] / 1e6 for k, v in kt.items()},
+           "alg_bytes": alg, "achieved_GBps": alg / (dev_ns * 1e-9) / 1e9,
+           "frac": alg / (dev_ns * 1e-9) / 1e9 / HBM_PEAK_GBS, "words_per_s": words / (dev_ns * 1e-9)}
+    if with_cpu and len(res["words"]):
+        k = min(len(res["words"]), 2_000_000)
+        c0 = time.perf_counter()
+        want = oracle.to_asm(res["words"][:k])
+        c1 = time.perf_counter()
+        out["cpu_baseline"] = {"value": k / (c1 - c0), "unit": "words/s", "cores": 1, "kind": "port",
+                               "sample": f"first {k} words, the oracle's to_asm restatement, {c1 - c0:.2f}s"}
+        out["bit_exact"] = text[:len(want)] == want
+    log(f"to_asm: {words} words -> {len(text)} bytes, device {dev_ns / 1e6:.2f} ms "
+        f"({out['achieved_GBps']:.0f} GB/s algorithmic), wall {wall * 1e3:.1f} ms"
+        + (f", host {out['cpu_baseline']['value'] / 1e6:.2f} M words/s, bit-exact {out['bit_exact']}"
+           if "cpu_baseline" in out else ""))
+    return out
 
 
 def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: int, cpu_nodes: int) -> dict:
@@ -380,6 +419,8 @@ def main() -> int:
     log("kernels (ms total, launches): " + ", ".join(
         f"{k}={v[0] / 1e6:.3f}/{v[1]}" for k, v in sorted(ktimes.items(), key=lambda kv: -kv[1][0])))
 
+    to_asm = measure_asm(ctx, res, not args.no_cpu_baseline)
+
     cpu = None
     bit_exact = None
     if not args.no_cpu_baseline:
@@ -413,7 +454,7 @@ def main() -> int:
                    "max_depth": tree.max_depth, "words": res["n_words"], "path": "plain one-program GPU pipeline",
                    "inputs": "resident in HBM"},
         "roofline": roof, "cpu_baseline": cpu, "bit_exact": bit_exact, "stages_ms": stages,
-        "stages_roofline": sroof,
+        "stages_roofline": sroof, "to_asm": to_asm,
         "batched": batched, "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -131,6 +131,20 @@ int ptl_server_release(ptl_server *s, uint64_t ticket);
 int ptl_server_stats(ptl_server *s, uint64_t *batches, uint64_t *requests_done, uint64_t *nodes_done,
                      uint64_t *largest_batch);
 
+/* Text output: replaces rv_generator::to_asm (src/codegen/rv_generator.cpp:114-120),
+ * which prints every word as rvdisasm::instruction(word) (src/disassembler.cpp:453-462)
+ * followed by '\n': the mnemonic, one space, the operands, exactly as the
+ * reference prints them (its switch fall-throughs included).  The lines are
+ * formatted on the device.  ptl_result_asm formats program `program` (0 for a
+ * single tree) of the last compile, whose words are still in HBM;
+ * ptl_disassemble formats host words (and, like a compile, replaces the
+ * context's previous results).  The text (not NUL-terminated) is owned by the
+ * context and valid until the next call on it.  PTL_ERR_ARG when there is no
+ * complete compile or program is out of range.  Raw words (to_binary,
+ * rv_generator.cpp:109-112) are ptl_result.words as they are. */
+int ptl_result_asm(ptl_ctx *ctx, uint32_t program, const char **text, uint64_t *n_bytes);
+int ptl_disassemble(ptl_ctx *ctx, const uint32_t *words, uint64_t n_words, const char **text, uint64_t *n_bytes);
+
 /* Host-side stage dumps for differential testing (stage 1..7 boundary arrays
  * of the last compile): name is one of "types", "rtypes", "node_data", "locs",
  * "func_ids", "func_starts", "func_ends", "func_sizes", "instr", "rd", "rs1",

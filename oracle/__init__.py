@@ -49,6 +49,8 @@ def lib():
         L.ptlo_ties.restype = C.c_uint64
         L.ptlo_exclusive_scan.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
         L.ptlo_introsort.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        L.ptlo_asm.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
+        L.ptlo_asm.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -124,6 +126,16 @@ def introsort(keys: np.ndarray) -> np.ndarray:
     idx = np.arange(k.shape[0], dtype=np.uint32)
     lib().ptlo_introsort(idx.ctypes.data, k.shape[0], k.ctypes.data)
     return idx
+
+
+def to_asm(words) -> bytes:
+    """rv_generator::to_asm text of the words (src/codegen/rv_generator.cpp:114-120)."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    out = np.zeros(max(1, 64 * w.shape[0]), np.uint8)
+    n = lib().ptlo_asm(w.ctypes.data, w.shape[0], out.ctypes.data, out.shape[0])
+    if n < 0:
+        raise RuntimeError("ptlo_asm: output buffer too small")
+    return out[:n].tobytes()
 
 
 def ref_front_end(src_path: str, out_path: str):

@@ -995,3 +995,180 @@ void ptlo_exclusive_scan(const uint32_t *in, uint32_t *out, int64_t n) {
 /* std::sort restated, exposed so a test can compare it with the host's own
  * libstdc++ std::sort on the same keys. */
 void ptlo_introsort(uint32_t *idx, int64_t n, const int32_t *key) { sort_introsort(idx, n, key); }
+
+/* ------------------------------------------------------------ text output --
+ * rv_generator::to_asm (src/codegen/rv_generator.cpp:114-120): every word as
+ * rvdisasm::instruction(word) (src/disassembler.cpp:453-462), then '\n'.  That
+ * is instr_name (:63-222), one space, and format_args (:224-407) with no colour
+ * and address 0.  The reference's switches fall through wherever an inner
+ * switch finds no match; C's own fall-through reproduces that here, case for
+ * case in the reference's source order. */
+static const char *const ASM_X[32] = {
+    "zero", "ra", "sp", "gp", "tp", "t0", "t1", "t2", "s0", "s1", "a0", "a1", "a2", "a3", "a4", "a5",
+    "a6", "a7", "s2", "s3", "s4", "s5", "s6", "s7", "s8", "s9", "s10", "s11", "t3", "t4", "t5", "t6"};
+static const char *const ASM_F[32] = {
+    "ft0", "ft1", "ft2", "ft3", "ft4", "ft5", "ft6", "ft7", "fs0", "fs1", "fa0", "fa1", "fa2", "fa3", "fa4", "fa5",
+    "fa6", "fa7", "fs2", "fs3", "fs4", "fs5", "fs6", "fs7", "fs8", "fs9", "fs10", "fs11", "ft8", "ft9", "ft10", "ft11"};
+
+/* instr_name, src/disassembler.cpp:83-222 */
+static const char *asm_name(uint32_t w) {
+    const uint32_t op = w & 0x7F, f3 = (w >> 12) & 7, f7 = (w >> 25) & 0x7F, r2 = (w >> 20) & 0x1F;
+    switch (op) {
+        case 0x37: return "lui";
+        case 0x17: return "auipc";
+        case 0x6F: return "jal";
+        case 0x67: return "jalr";
+        case 0x63:
+            switch (f3) { case 0: return "beq"; case 1: return "bne"; case 4: return "blt"; case 5: return "bge";
+                          case 6: return "bltu"; case 7: return "bgeu"; }
+            /* fall through */
+        case 0x03:
+            switch (f3) { case 0: return "lb"; case 1: return "lh"; case 2: return "lw"; case 4: return "lbu";
+                          case 5: return "lhu"; }
+            /* fall through */
+        case 0x23:
+            switch (f3) { case 0: return "sb"; case 1: return "sh"; case 2: return "sw"; }
+            /* fall through */
+        case 0x13:
+            switch (f3) {
+                case 0: return "addi"; case 1: return "slli"; case 2: return "slti"; case 3: return "sltiu";
+                case 4: return "xori";
+                case 5: if (f7 == 0x00) return "srli"; if (f7 == 0x20) return "srai"; /* fall through */
+                case 6: return "ori";
+                case 7: return "andi";
+            }
+            /* fall through */
+        case 0x33:
+            switch (f3) {
+                case 0: if (f7 == 0) return "add"; if (f7 == 1) return "mul"; if (f7 == 0x20) return "sub"; /* fall through */
+                case 1: if (f7 == 0) return "sll"; if (f7 == 1) return "mulh"; /* fall through */
+                case 2: if (f7 == 0) return "slt"; if (f7 == 1) return "mulhsu"; /* fall through */
+                case 3: if (f7 == 0) return "sltu"; if (f7 == 1) return "mulhu"; /* fall through */
+                case 4: if (f7 == 0) return "xor"; if (f7 == 1) return "div"; /* fall through */
+                case 5: if (f7 == 0) return "srl"; if (f7 == 1) return "divu"; if (f7 == 0x20) return "sra"; /* fall through */
+                case 6: if (f7 == 0) return "or"; if (f7 == 1) return "rem"; /* fall through */
+                case 7: if (f7 == 0) return "and"; if (f7 == 1) return "remu";
+            }
+            /* fall through */
+        case 0x0F: return "fence";
+        case 0x73:
+            if (f7 == 0) return "ecall";
+            if (f7 == 1) return "ebreak";
+            /* fall through */
+        case 0x07: return "flw";
+        case 0x27: return "fsw";
+        case 0x43: return "fmadd.s";
+        case 0x47: return "fmsub.s";
+        case 0x4B: return "fnmsub.s";
+        case 0x4F: return "fnmadd.s";
+        case 0x53:
+            switch (f7) {
+                case 0x00: return "fadd.s";
+                case 0x04: return "fsub.s";
+                case 0x08: return "fmul.s";
+                case 0x0C: return "fdiv.s";
+                case 0x2C: return "fsqrt.s";
+                case 0x10: if (f3 == 0) return "fsgnj.s"; if (f3 == 1) return "fsgnjn.s"; if (f3 == 2) return "fsgnjx.s"; /* fall through */
+                case 0x14: if (f3 == 0) return "fmin.s"; if (f3 == 1) return "fmax.s"; /* fall through */
+                case 0x60: if (r2 == 0) return "fcvt.w.s"; if (r2 == 1) return "fcvt.wu.s"; /* fall through */
+                case 0x70: if (f3 == 0) return "fmv.x.w"; if (f3 == 1) return "fclass.s"; /* fall through */
+                case 0x50: if (f3 == 0) return "fle.s"; if (f3 == 1) return "flt.s"; if (f3 == 2) return "feq.s"; /* fall through */
+                case 0x68: if (r2 == 0) return "fcvt.s.w"; if (r2 == 1) return "fcvt.s.wu"; /* fall through */
+                case 0x78: return "fmv.w.x";
+            }
+    }
+    return w ? "unknown" : "(zero)";
+}
+
+/* instr_type, :56-81 */
+enum { AT_UNKNOWN, AT_R, AT_I, AT_S, AT_B, AT_U, AT_J, AT_R4 };
+static int asm_type(uint32_t w) {
+    switch (w & 0x7F) {
+        case 0x37: case 0x17: return AT_U;
+        case 0x6F: return AT_J;
+        case 0x67: case 0x03: case 0x13: case 0x0F: case 0x73: case 0x07: return AT_I;
+        case 0x63: return AT_B;
+        case 0x23: case 0x27: return AT_S;
+        case 0x33: case 0x53: return AT_R;
+        case 0x43: case 0x47: case 0x4B: case 0x4F: return AT_R4;
+    }
+    return AT_UNKNOWN;
+}
+
+/* One line: instr_name, ' ', format_args (:224-407, no colour, address 0), '\n'.
+ * Returns its length; out needs 64 bytes. */
+static int asm_line(uint32_t w, char *out) {
+    const uint32_t op = w & 0x7F, f3 = (w >> 12) & 7, f7 = (w >> 25) & 0x7F;
+    const uint32_t rd = (w >> 7) & 31, rs1 = (w >> 15) & 31, rs2 = (w >> 20) & 31, rs3 = (w >> 27) & 31;
+    int n = sprintf(out, "%s ", asm_name(w));
+    char *p = out + n;
+    switch (asm_type(w)) {
+        case AT_R:
+            if (op == 0x53) {
+                if (f7 == 0x78 || f7 == 0x68) p += sprintf(p, "%s, %s", ASM_F[rd], ASM_X[rs1]);
+                else if (f7 == 0x60 || (f7 == 0x70 && f3 == 0)) p += sprintf(p, "%s, %s", ASM_X[rd], ASM_F[rs1]);
+                else p += sprintf(p, "%s, %s, %s", ASM_F[rd], ASM_F[rs1], ASM_F[rs2]);
+            } else {
+                p += sprintf(p, "%s, %s, %s", ASM_X[rd], ASM_X[rs1], ASM_X[rs2]);
+            }
+            break;
+        case AT_I: {
+            uint32_t imm = w >> 20;
+            if (imm >> 11) imm |= 0xFFFFF000u;
+            if (op == 0x07 || op == 0x03 || op == 0x67)
+                p += sprintf(p, "%s, %d(%s)", op == 0x07 ? ASM_F[rd] : ASM_X[rd], (int32_t)imm, ASM_X[rs1]);
+            else
+                p += sprintf(p, "%s, %s, %d", ASM_X[rd], ASM_X[rs1], (int32_t)imm);
+            break;
+        }
+        case AT_S: {
+            uint32_t imm = ((w >> 7) & 0x1F) | ((w >> 20) & 0xFE0);
+            if (imm >> 11) imm |= 0xFFFFF000u;
+            /* an fsw names its register by the rs1 field (:357-361) */
+            p += sprintf(p, "%s, %d(%s)", op == 0x27 ? ASM_F[rs1] : ASM_X[rs2], (int32_t)imm, ASM_X[rs1]);
+            break;
+        }
+        case AT_B: {
+            /* bits 4:1, 9:5 (instr[29:25]: bit 10 is not taken) and 11; the
+             * sign fills bits 31:11 (:366-372) */
+            uint32_t imm = ((w >> 7) & 0x1E) | ((w >> 20) & 0x3E0) | (((w >> 7) & 1u) << 11);
+            if (w >> 31) imm |= 0xFFFFF800u;
+            const uint64_t tgt = (uint64_t)(int64_t)(int32_t)imm;  /* address 0 + offset, as uint64 */
+            if (tgt) p += sprintf(p, "%s, %s, 0x%llx", ASM_X[rs1], ASM_X[rs2], (unsigned long long)tgt);
+            else p += sprintf(p, "%s, %s, 0", ASM_X[rs1], ASM_X[rs2]);  /* showbase prints 0 bare */
+            break;
+        }
+        case AT_U:
+            p += sprintf(p, "%s, %d", ASM_X[rd], (int32_t)(w & 0xFFFFF000u));
+            break;
+        case AT_J: {
+            /* bits 10:1, 11, 19:12; the sign fills bits 31:19 (:384-392) */
+            uint32_t imm = ((w >> 20) & 0x7FE) | (((w >> 20) & 1u) << 11) | (w & 0xFF000u);
+            if (w >> 31) imm |= 0xFFF80000u;
+            p += sprintf(p, "%s, %d", ASM_X[rd], (int32_t)imm);
+            break;
+        }
+        case AT_R4:
+            p += sprintf(p, "%s, %s, %s, %s", ASM_X[rd], ASM_X[rs1], ASM_X[rs2], ASM_X[rs3]);
+            break;
+        default:
+            break;
+    }
+    *p++ = '\n';
+    *p = 0;
+    return (int)(p - out);
+}
+
+/* to_asm of n words into out (cap bytes); returns the text length, or -1 when
+ * it does not fit. */
+int64_t ptlo_asm(const uint32_t *words, int64_t n, char *out, int64_t cap) {
+    int64_t len = 0;
+    char line[64];
+    for (int64_t i = 0; i < n; ++i) {
+        const int k = asm_line(words[i], line);
+        if (len + k > cap) return -1;
+        memcpy(out + len, line, (size_t)k);
+        len += k;
+    }
+    return len;
+}

@@ -176,19 +176,18 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t *sums, uint32_t nt
     if (threadIdx.x == 0 && total) *total = s_carry;
 }
 
+// The rescan of one tile: v[k] = element base + k of this thread (base = tile
+// start + 16 * thread), and the exclusive prefix of its first element.
 template <class Op>
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(Op op, uint64_t n, const uint32_t *tile_pre) {
-    __shared__ uint32_t s_w[SCAN_BLOCK / 64];
-    __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 16];  // element e at e + e/16 (one pad word per 16)
+DEV uint32_t scan_tile(const Op &op, uint64_t n, const uint32_t *tile_pre, uint32_t *s_w, uint32_t *s_v,
+                       uint32_t (&v)[SCAN_ITEMS]) {
     const uint64_t tb = (uint64_t)blockIdx.x * SCAN_TILE;
-    const uint64_t base = tb + (uint64_t)threadIdx.x * SCAN_ITEMS;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         const uint32_t e = (uint32_t)k * SCAN_BLOCK + threadIdx.x;
-        s_v[e + e / 16u] = tb + e < n ? op.value(tb + e) : 0u;
+        s_v[e + e / 16u] = tb + e < n ? op.value(tb + e) : 0u;  // element e at e + e/16 (one pad word per 16)
     }
     __syncthreads();
-    uint32_t v[SCAN_ITEMS];
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -202,6 +201,16 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(Op op, uint64_t n, con
     uint32_t run = tile_pre[blockIdx.x] + x - sum;
 #pragma unroll
     for (int w = 0; w < SCAN_BLOCK / 64; ++w) if ((uint32_t)w < wid) run += s_w[w];
+    return run;
+}
+
+template <class Op>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(Op op, uint64_t n, const uint32_t *tile_pre) {
+    __shared__ uint32_t s_w[SCAN_BLOCK / 64];
+    __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 16];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t run = scan_tile(op, n, tile_pre, s_w, s_v, v);
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         uint64_t i = base + k;
@@ -2223,6 +2232,264 @@ __global__ __launch_bounds__(256) void k_fj_move(Dev d) {
         return;
     }
     d.words[k] = w | pp;
+}
+
+// ============================================================== text output
+// rv_generator::to_asm (src/codegen/rv_generator.cpp:114-120): one line per
+// word, rvdisasm::instruction(word) (src/disassembler.cpp:453-462) and '\n':
+// the mnemonic (instr_name, :83-222), a space, the operands (format_args,
+// :224-407, no colour, address 0).  The reference's switches continue with the
+// next case in source order wherever an inner switch finds no match; asm_name
+// walks its cases as such a chain, so a branch with funct3 2 is named "lw" and
+// a SYSTEM word other than ecall/ebreak "flw", exactly as the reference prints
+// them.  Lines are at most 48 bytes.  Each word is formatted by one thread: a
+// scan of the line lengths gives every line's offset, and the rescan writes
+// each thread's 16 consecutive lines (4-byte stores inside its own byte range,
+// byte stores at its two ends).
+enum : uint8_t {
+    M_LUI, M_AUIPC, M_JAL, M_JALR, M_BEQ, M_BNE, M_BLT, M_BGE, M_BLTU, M_BGEU, M_LB, M_LH, M_LW, M_LBU, M_LHU,
+    M_SB, M_SH, M_SW, M_ADDI, M_SLLI, M_SLTI, M_SLTIU, M_XORI, M_SRLI, M_SRAI, M_ORI, M_ANDI, M_ADD, M_MUL,
+    M_SUB, M_SLL, M_MULH, M_SLT, M_MULHSU, M_SLTU, M_MULHU, M_XOR, M_DIV, M_SRL, M_DIVU, M_SRA, M_OR, M_REM,
+    M_AND, M_REMU, M_FENCE, M_ECALL, M_EBREAK, M_FLW, M_FSW, M_FMADD, M_FMSUB, M_FNMSUB, M_FNMADD, M_FADD,
+    M_FSUB, M_FMUL, M_FDIV, M_FSQRT, M_FSGNJ, M_FSGNJN, M_FSGNJX, M_FMIN, M_FMAX, M_FCVT_W_S, M_FCVT_WU_S,
+    M_FMV_X_W, M_FCLASS, M_FLE, M_FLT, M_FEQ, M_FCVT_S_W, M_FCVT_S_WU, M_FMV_W_X, M_UNKNOWN, M_ZERO, M_COUNT
+};
+// names and registers in 16- and 8-byte slots, read as 8-byte words
+__constant__ __align__(16) char c_asm_name[M_COUNT][16] = {
+    "lui", "auipc", "jal", "jalr", "beq", "bne", "blt", "bge", "bltu", "bgeu", "lb", "lh", "lw", "lbu", "lhu",
+    "sb", "sh", "sw", "addi", "slli", "slti", "sltiu", "xori", "srli", "srai", "ori", "andi", "add", "mul",
+    "sub", "sll", "mulh", "slt", "mulhsu", "sltu", "mulhu", "xor", "div", "srl", "divu", "sra", "or", "rem",
+    "and", "remu", "fence", "ecall", "ebreak", "flw", "fsw", "fmadd.s", "fmsub.s", "fnmsub.s", "fnmadd.s",
+    "fadd.s", "fsub.s", "fmul.s", "fdiv.s", "fsqrt.s", "fsgnj.s", "fsgnjn.s", "fsgnjx.s", "fmin.s", "fmax.s",
+    "fcvt.w.s", "fcvt.wu.s", "fmv.x.w", "fclass.s", "fle.s", "flt.s", "feq.s", "fcvt.s.w", "fcvt.s.wu",
+    "fmv.w.x", "unknown", "(zero)"};
+// ABI register names (:15-50)
+__constant__ __align__(8) char c_asm_x[32][8] = {"zero", "ra", "sp", "gp", "tp", "t0", "t1", "t2", "s0", "s1", "a0",
+                                    "a1", "a2", "a3", "a4", "a5", "a6", "a7", "s2", "s3", "s4", "s5",
+                                    "s6", "s7", "s8", "s9", "s10", "s11", "t3", "t4", "t5", "t6"};
+__constant__ __align__(8) char c_asm_f[32][8] = {"ft0", "ft1", "ft2", "ft3", "ft4", "ft5", "ft6", "ft7", "fs0", "fs1", "fa0",
+                                    "fa1", "fa2", "fa3", "fa4", "fa5", "fa6", "fa7", "fs2", "fs3", "fs4", "fs5",
+                                    "fs6", "fs7", "fs8", "fs9", "fs10", "fs11", "ft8", "ft9", "ft10", "ft11"};
+
+// instr_name: the case chain entered at the word's opcode.
+DEV uint32_t asm_name(uint32_t w) {
+    const uint32_t op = w & 0x7Fu, f3 = (w >> 12) & 7u, f7 = (w >> 25) & 0x7Fu, r2 = (w >> 20) & 0x1Fu;
+    // cases in source order: 0 lui, 1 auipc, 2 jal, 3 jalr, 4 BRANCH, 5 LOAD,
+    // 6 STORE, 7 OP-IMM, 8 OP, 9 fence, 10 SYSTEM, 11 flw, 12 fsw, 13-16 the
+    // fused multiply-adds, 17 OP-FP
+    int k;
+    switch (op) {
+        case 0x37: k = 0; break;  case 0x17: k = 1; break;  case 0x6F: k = 2; break;  case 0x67: k = 3; break;
+        case 0x63: k = 4; break;  case 0x03: k = 5; break;  case 0x23: k = 6; break;  case 0x13: k = 7; break;
+        case 0x33: k = 8; break;  case 0x0F: k = 9; break;  case 0x73: k = 10; break; case 0x07: k = 11; break;
+        case 0x27: k = 12; break; case 0x43: k = 13; break; case 0x47: k = 14; break; case 0x4B: k = 15; break;
+        case 0x4F: k = 16; break; case 0x53: k = 17; break;
+        default: return w ? M_UNKNOWN : M_ZERO;
+    }
+    for (; k < 18; ++k) {
+        switch (k) {
+            case 0: return M_LUI;
+            case 1: return M_AUIPC;
+            case 2: return M_JAL;
+            case 3: return M_JALR;
+            case 4: { constexpr uint8_t t[8] = {M_BEQ, M_BNE, 0xFF, 0xFF, M_BLT, M_BGE, M_BLTU, M_BGEU};
+                      if (t[f3] != 0xFF) return t[f3]; break; }
+            case 5: { constexpr uint8_t t[8] = {M_LB, M_LH, M_LW, 0xFF, M_LBU, M_LHU, 0xFF, 0xFF};
+                      if (t[f3] != 0xFF) return t[f3]; break; }
+            case 6: if (f3 < 3) return M_SB + f3; break;
+            case 7: { constexpr uint8_t t[8] = {M_ADDI, M_SLLI, M_SLTI, M_SLTIU, M_XORI, 0xFF, M_ORI, M_ANDI};
+                      if (f3 == 5) return f7 == 0 ? M_SRLI : f7 == 0x20 ? M_SRAI : M_ORI;  // else into funct3 6
+                      return t[f3]; }
+            case 8: {
+                // funct3 cases from the word's own on, each matching funct7 0, 1
+                // (and 0x20 for add/sub, srl/sra), else the next funct3 case
+                constexpr uint8_t base[8] = {M_ADD, M_SLL, M_SLT, M_SLTU, M_XOR, M_SRL, M_OR, M_AND};
+                constexpr uint8_t mext[8] = {M_MUL, M_MULH, M_MULHSU, M_MULHU, M_DIV, M_DIVU, M_REM, M_REMU};
+                for (uint32_t g = f3; g < 8; ++g) {
+                    if (f7 == 0) return base[g];
+                    if (f7 == 1) return mext[g];
+                    if (f7 == 0x20 && g == 0) return M_SUB;
+                    if (f7 == 0x20 && g == 5) return M_SRA;
+                }
+                break;
+            }
+            case 9: return M_FENCE;
+            case 10: if (f7 == 0) return M_ECALL; if (f7 == 1) return M_EBREAK; break;
+            case 11: return M_FLW;
+            case 12: return M_FSW;
+            case 13: return M_FMADD;
+            case 14: return M_FMSUB;
+            case 15: return M_FNMSUB;
+            case 16: return M_FNMADD;
+            case 17: {
+                // funct7 cases in source order, entered at the word's own
+                constexpr uint8_t sev[12] = {0x00, 0x04, 0x08, 0x0C, 0x2C, 0x10, 0x14, 0x60, 0x70, 0x50, 0x68, 0x78};
+                uint32_t t = 0;
+                while (t < 12 && sev[t] != f7) ++t;
+                for (; t < 12; ++t) {
+                    switch (t) {
+                        case 0: return M_FADD;
+                        case 1: return M_FSUB;
+                        case 2: return M_FMUL;
+                        case 3: return M_FDIV;
+                        case 4: return M_FSQRT;
+                        case 5: if (f3 < 3) return M_FSGNJ + f3; break;
+                        case 6: if (f3 < 2) return M_FMIN + f3; break;
+                        case 7: if (r2 < 2) return M_FCVT_W_S + r2; break;
+                        case 8: if (f3 < 2) return M_FMV_X_W + f3; break;
+                        case 9: if (f3 < 3) return M_FLE + f3; break;
+                        case 10: if (r2 < 2) return M_FCVT_S_W + r2; break;
+                        default: return M_FMV_W_X;
+                    }
+                }
+                break;
+            }
+        }
+    }
+    return w ? M_UNKNOWN : M_ZERO;
+}
+
+// the bytes of v up to the first zero; true when all eight were emitted
+template <class S> DEV bool asm_8(S &s, uint64_t v) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const char c = (char)(v >> (8 * i));
+        if (!c) return false;
+        s.put(c);
+    }
+    return true;
+}
+template <class S> DEV void asm_str16(S &s, const char *p) {  // a 16-byte slot
+    const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
+    const uint64_t lo = q[0], hi = q[1];
+    if (asm_8(s, lo)) asm_8(s, hi);
+}
+template <class S> DEV void asm_str8(S &s, const char *p) { asm_8(s, *reinterpret_cast<const uint64_t *>(p)); }
+template <class S> DEV void asm_dec(S &s, int32_t x) {
+    uint32_t u = (uint32_t)x;
+    if (x < 0) { s.put('-'); u = 0u - u; }
+    uint64_t dig = 0;  // decimal digits, 4 bits each, least significant first
+    int nd = 0;
+    do { dig |= (uint64_t)(u % 10u) << (4 * nd); u /= 10u; ++nd; } while (u);
+    for (int k = nd - 1; k >= 0; --k) s.put((char)('0' + ((dig >> (4 * k)) & 15u)));
+}
+// std::hex << std::showbase: "0x" and the digits, a bare "0" for zero
+template <class S> DEV void asm_hex(S &s, uint64_t v) {
+    if (!v) { s.put('0'); return; }
+    s.put('0'); s.put('x');
+    for (int k = (67 - __clzll((long long)v)) / 4 - 1; k >= 0; --k) {
+        const uint32_t d = (uint32_t)(v >> (4 * k)) & 15u;
+        s.put((char)(d < 10 ? '0' + d : 'a' + d - 10));
+    }
+}
+
+template <class S> DEV void asm_line(S &s, uint32_t w) {
+    const uint32_t op = w & 0x7Fu, f3 = (w >> 12) & 7u, f7 = (w >> 25) & 0x7Fu;
+    const uint32_t rd = (w >> 7) & 31u, rs1 = (w >> 15) & 31u, rs2 = (w >> 20) & 31u, rs3 = (w >> 27) & 31u;
+    asm_str16(s, c_asm_name[asm_name(w)]);
+    s.put(' ');
+    auto X = [&](uint32_t r) { asm_str8(s, c_asm_x[r]); };
+    auto F = [&](uint32_t r) { asm_str8(s, c_asm_f[r]); };
+    auto sep = [&]() { s.put(','); s.put(' '); };
+    switch (op) {
+        case 0x33:  // R
+            X(rd); sep(); X(rs1); sep(); X(rs2);
+            break;
+        case 0x53:  // R, OP-FP: register files by funct7 (:250-281)
+            if (f7 == 0x78 || f7 == 0x68) { F(rd); sep(); X(rs1); }
+            else if (f7 == 0x60 || (f7 == 0x70 && f3 == 0)) { X(rd); sep(); F(rs1); }
+            else { F(rd); sep(); F(rs1); sep(); F(rs2); }
+            break;
+        case 0x67: case 0x03: case 0x13: case 0x0F: case 0x73: case 0x07: {  // I
+            const int32_t imm = (int32_t)w >> 20;
+            if (op == 0x07 || op == 0x03 || op == 0x67) {
+                if (op == 0x07) F(rd); else X(rd);
+                sep(); asm_dec(s, imm); s.put('('); X(rs1); s.put(')');
+            } else {
+                X(rd); sep(); X(rs1); sep(); asm_dec(s, imm);
+            }
+            break;
+        }
+        case 0x23: case 0x27: {  // S; an fsw names its register by the rs1 field (:357-361)
+            const int32_t imm = ((int32_t)(w & 0xFE000000u) >> 20) | (int32_t)((w >> 7) & 0x1Fu);
+            if (op == 0x27) F(rs1); else X(rs2);
+            sep(); asm_dec(s, imm); s.put('('); X(rs1); s.put(')');
+            break;
+        }
+        case 0x63: {  // B: bits 4:1, 9:5 (not 10) and 11, the sign over bits 31:11 (:366-372); at address 0
+            uint32_t imm = ((w >> 7) & 0x1Eu) | ((w >> 20) & 0x3E0u) | (((w >> 7) & 1u) << 11);
+            if (w >> 31) imm |= 0xFFFFF800u;
+            X(rs1); sep(); X(rs2); sep(); asm_hex(s, (uint64_t)(int64_t)(int32_t)imm);
+            break;
+        }
+        case 0x37: case 0x17:  // U
+            X(rd); sep(); asm_dec(s, (int32_t)(w & 0xFFFFF000u));
+            break;
+        case 0x6F: {  // J: bits 10:1, 11, 19:12, the sign over bits 31:19 (:384-392)
+            uint32_t imm = ((w >> 20) & 0x7FEu) | (((w >> 20) & 1u) << 11) | (w & 0xFF000u);
+            if (w >> 31) imm |= 0xFFF80000u;
+            X(rd); sep(); asm_dec(s, (int32_t)imm);
+            break;
+        }
+        case 0x43: case 0x47: case 0x4B: case 0x4F:  // R4 (integer names, :395-403)
+            X(rd); sep(); X(rs1); sep(); X(rs2); sep(); X(rs3);
+            break;
+        default:
+            break;
+    }
+    s.put('\n');
+}
+
+struct AsmCount {
+    uint32_t n = 0;
+    DEV void put(char) { ++n; }
+};
+
+// Writes bytes from pos on; a 4-byte word lying inside [lo, hi) (the thread's
+// own lines) goes out as one store, the bytes of a word shared with a
+// neighbour's lines one by one.
+struct AsmWrite {
+    uint8_t *out;
+    uint64_t pos, lo, hi;
+    uint32_t acc = 0;
+    DEV void put(char ch) {
+        const uint64_t wb = pos & ~3ull;
+        if (wb >= lo && wb + 4 <= hi) {
+            acc |= (uint32_t)(uint8_t)ch << (8u * (uint32_t)(pos & 3u));
+            if ((pos & 3u) == 3u) { *reinterpret_cast<uint32_t *>(out + wb) = acc; acc = 0; }
+        } else {
+            out[pos] = (uint8_t)ch;
+        }
+        ++pos;
+    }
+};
+
+// line lengths: formatted once with a counting sink (the tile sums), kept as
+// one byte per word for the rescan
+struct AsmLenOp {
+    const uint32_t *w;
+    uint8_t *len;
+    DEV uint32_t value(uint64_t i) const { AsmCount c; asm_line(c, w[i]); len[i] = (uint8_t)c.n; return c.n; }
+    DEV void emit(uint64_t, uint32_t, uint32_t) const {}
+};
+struct AsmLenRead {
+    const uint8_t *len;
+    DEV uint32_t value(uint64_t i) const { return len[i]; }
+};
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_asm_write(const uint32_t *w, const uint8_t *len8, uint64_t n,
+                                                          const uint32_t *tile_pre, uint8_t *out) {
+    __shared__ uint32_t s_w[SCAN_BLOCK / 64];
+    __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 16];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    const uint32_t run = scan_tile(AsmLenRead{len8}, n, tile_pre, s_w, s_v, v);
+    uint32_t len = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) len += v[k];
+    AsmWrite s{out, run, run, (uint64_t)run + len};
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+        if (base + k < n) asm_line(s, w[base + k]);
 }
 
 // ================================================================ host side

@@ -129,6 +129,22 @@ class Context:
     def stop_after(self, stage: int) -> None:
         self.lib.ptl_debug_stop_after(self.h, stage)
 
+    def _text(self, s, t, n) -> bytes:
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return C.string_at(t.value, n.value) if n.value else b""
+
+    def result_asm(self, program: int = 0) -> bytes:
+        """to_asm text of program `program` of the last compile, formatted on the device."""
+        t, n = C.c_void_p(), C.c_uint64()
+        return self._text(self.lib.ptl_result_asm(self.h, program, C.byref(t), C.byref(n)), t, n)
+
+    def disassemble(self, words) -> bytes:
+        """to_asm text of host words, formatted on the device."""
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        t, n = C.c_void_p(), C.c_uint64()
+        return self._text(self.lib.ptl_disassemble(self.h, w.ctypes.data, w.shape[0], C.byref(t), C.byref(n)), t, n)
+
     def set_overlap(self, on) -> None:
         """Walk and lockstep at the same time (True, default) or one after the
         other (False); 2 forces the lockstep's bounded wait to give up and the
@@ -257,3 +273,13 @@ class RvGeneratorHip:
     def to_binary(self, stream) -> None:
         """Raw little-endian words (rv_generator.cpp:116-119)."""
         stream.write(self.get_instructions().astype("<u4").tobytes())
+
+    def to_asm(self, stream) -> None:
+        """One line per word, rvdisasm::instruction(word) and a newline
+        (rv_generator.cpp:114-120), formatted on the device.  Text streams get
+        str, binary streams bytes."""
+        txt = self.ctx.disassemble(self.get_instructions())
+        try:
+            stream.write(txt)
+        except TypeError:
+            stream.write(txt.decode("ascii"))
