@@ -433,22 +433,43 @@ struct Lane {
     uint32_t dpv, loc, word, rd, rs1, rs2, jt, wb;
 };
 
+// The inputs of one node that all four of its lanes share, loaded once: its
+// program (word base, function base and count), types, parent, parent type,
+// child index, location and data.
+struct GenNode {
+    uint32_t nn, prg, wb, fb, Fp, wn, ty, rt, ci, loc, ndv, pty;
+    int32_t par;
+};
+
+DEV GenNode gen_node(const Dev &d, uint32_t nn) {
+    GenNode N;
+    N.nn = nn;
+    N.prg = prog_of(d.pn, d.P, nn);
+    N.wb = d.pw[N.prg]; N.wn = d.pw[N.prg + 1] - N.wb;
+    N.fb = d.pf[N.prg]; N.Fp = d.pf[N.prg + 1] - N.fb;
+    N.ty = d.ty8[nn]; N.rt = d.rt8[nn];
+    N.par = d.par[nn];
+    N.pty = N.par != -1 ? d.ty8[N.par] : 0u;
+    N.ci = (uint32_t)d.cidx[nn];
+    N.loc = d.loc[nn];
+    N.ndv = d.nd[nn];
+    return N;
+}
+
 // One (node, offset) lane of the level loop body (:585-884), computed in the
 // word indices of the node's own program (register numbers are ORed with word
 // indices, :676-697, :842); L.loc is the global word slot.
-DEV void gen_lane(const Dev &d, uint32_t nn, uint32_t o, bool full, Lane &L) {
-    const uint32_t prg = prog_of(d.pn, d.P, nn);
-    const uint32_t wb = d.pw[prg], fb = d.pf[prg], Fp = d.pf[prg + 1] - fb;
-    const uint32_t ty = d.ty8[nn], rt = d.rt8[nn];
+DEV void gen_lane(const Dev &d, const GenNode &N, uint32_t o, bool full, Lane &L) {
+    const uint32_t nn = N.nn, wb = N.wb, fb = N.fb, Fp = N.Fp, ty = N.ty, rt = N.rt;
     const uint32_t si = spec_i(ty, o, rt);
     const uint32_t b = c_bits[si];
     const bool has = b & 1u;
     const bool prop = has || o == 0;
-    const int32_t par = prop ? d.par[nn] : 0;
+    const int32_t par = prop ? N.par : 0;
     const bool nopar = par == -1;
     const bool lfp = !nopar && prop;
-    const uint32_t pty = lfp ? d.ty8[par] : 0u;
-    const uint32_t ci = lfp ? (uint32_t)d.cidx[nn] : 0u;
+    const uint32_t pty = lfp ? N.pty : 0u;
+    const uint32_t ci = lfp ? N.ci : 0u;
     const bool noncond = ((pty & 0xD0u) == 0xD0u) && !(ci == (pty & 1u));
     bool call = noncond;
     if ((nopar || !noncond) && prop) {
@@ -458,9 +479,9 @@ DEV void gen_lane(const Dev &d, uint32_t nn, uint32_t o, bool full, Lane &L) {
     }
     L.has = has;
     L.pai = call ? (int32_t)((uint32_t)par * 3u + ci) : -1;
-    const uint32_t iib = o + (prop ? d.loc[nn] - wb : 0u);
+    const uint32_t iib = o + (prop ? N.loc - wb : 0u);
     uint32_t rd = 0;
-    const uint32_t ndv = d.nd[nn];
+    const uint32_t ndv = N.ndv;
     if (has) {
         uint32_t ct = (b >> 4) & 7u;
         if (ct == 3) rd |= 32u;
@@ -523,9 +544,10 @@ __global__ __launch_bounds__(256) void k_gen_regs(Dev d) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= d.n) return;
     int32_t slot = -1; uint32_t val = 0;
+    const GenNode N = gen_node(d, (uint32_t)i);
     for (uint32_t o = 0; o < 4; ++o) {
         Lane L;
-        gen_lane(d, (uint32_t)i, o, false, L);
+        gen_lane(d, N, o, false, L);
         if (L.pai >= 0) { slot = L.pai; val = L.dpv; }
     }
     if (slot >= 0) {
@@ -543,12 +565,14 @@ DEV uint64_t owner_key(const Dev &d, uint32_t nn, uint32_t o, bool use_rank) {
 __global__ __launch_bounds__(256) void k_gen_owner(Dev d, int use_rank) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= d.n) return;
+    const GenNode N = gen_node(d, (uint32_t)i);
+    const uint64_t k0 = owner_key(d, (uint32_t)i, 0, use_rank != 0);
     for (uint32_t o = 0; o < 4; ++o) {
         Lane L;
-        gen_lane(d, (uint32_t)i, o, true, L);
+        gen_lane(d, N, o, true, L);
         if (!L.has) continue;
         if (L.loc >= d.W0) { atomicOr(d.flags, EF_OOB); continue; }
-        atomicMax((unsigned long long *)&d.owner[L.loc], (unsigned long long)owner_key(d, (uint32_t)i, o, use_rank != 0));
+        atomicMax((unsigned long long *)&d.owner[L.loc], (unsigned long long)(k0 | o));
     }
 }
 
@@ -556,11 +580,13 @@ __global__ __launch_bounds__(256) void k_gen_owner(Dev d, int use_rank) {
 __global__ __launch_bounds__(256) void k_gen_ties(Dev d) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= d.n) return;
+    const GenNode N = gen_node(d, (uint32_t)i);
+    const uint64_t k0 = owner_key(d, (uint32_t)i, 0, false);
     for (uint32_t o = 0; o < 4; ++o) {
         Lane L;
-        gen_lane(d, (uint32_t)i, o, true, L);
+        gen_lane(d, N, o, true, L);
         if (!L.has || L.loc >= d.W0) continue;
-        uint64_t k = owner_key(d, (uint32_t)i, o, false);
+        const uint64_t k = k0 | o;
         uint64_t w = d.owner[L.loc];
         if ((w >> 34) == (k >> 34) && ((w >> 2) & 0xFFFFFFFFull) != i) atomicAdd(d.counter, 1u);
     }
@@ -569,15 +595,17 @@ __global__ __launch_bounds__(256) void k_gen_ties(Dev d) {
 __global__ __launch_bounds__(256) void k_gen_write(Dev d, int use_rank) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= d.n) return;
+    const GenNode N = gen_node(d, (uint32_t)i);
+    const uint64_t k0 = owner_key(d, (uint32_t)i, 0, use_rank != 0);
     for (uint32_t o = 0; o < 4; ++o) {
         Lane L;
-        gen_lane(d, (uint32_t)i, o, true, L);
+        gen_lane(d, N, o, true, L);
         if (!L.has || L.loc >= d.W0) continue;
-        if (d.owner[L.loc] != owner_key(d, (uint32_t)i, o, use_rank != 0)) continue;
+        if (d.owner[L.loc] != (k0 | o)) continue;
         // virtual registers (> 63) and targets become global word indices; a
         // virtual register past its program's words has no state (the
         // single-program check is virt_ok's)
-        const uint32_t wn = d.pw[prog_of(d.pn, d.P, (uint32_t)i) + 1] - L.wb;
+        const uint32_t wn = N.wn;
         if (d.P > 1 && (((int32_t)L.rd > 63 && L.rd - 64u >= wn) || ((int32_t)L.rs1 > 63 && L.rs1 - 64u >= wn) ||
                         ((int32_t)L.rs2 > 63 && L.rs2 - 64u >= wn)))
             atomicOr(d.flags, EF_OOB);
