@@ -1159,9 +1159,10 @@ DEV void ra_commit_syms(const Dev &d, const RaStep &r) {
 // permw lists these functions by size, largest first, so that the 64 walks of
 // a workgroup take about the same number of steps.  A function's virtual
 // registers are named by that function alone (k_xref_check), so each walk keeps
-// them in a write-back LDS cache of WALK_CACHE entries (slot = word offset in
-// the function mod WALK_CACHE; entry = offset << 10 | valid << 9 | state) and
-// touches HBM only on a miss or an eviction.  The words themselves are staged
+// them in an LDS cache of WALK_CACHE entries (slot = word offset in the
+// function mod WALK_CACHE; entry = offset << 10 | valid << 9 | state), reads
+// HBM only on a miss, and writes every change through to HBM (stores that are
+// never waited for).  The words themselves are staged
 // into LDS WALK_CHUNK steps at a time, loaded cooperatively (consecutive
 // threads, consecutive words of one function).
 constexpr uint32_t WALK_CACHE = 64;
@@ -1214,24 +1215,19 @@ DEV void wc_ring_set(WcRing &g, uint32_t k, uint32_t v) {
     g.r4 = k == 4 ? v : g.r4; g.r5 = k == 5 ? v : g.r5; g.r6 = k == 6 ? v : g.r6; g.r7 = k == 7 ? v : g.r7;
 }
 
-DEV void wc_evict(const Dev &d, uint32_t start, uint32_t e) {
-    if (e) d.sym[start + (e >> 10)] = (uint16_t)(e & 0x1FFu);
-}
-
-DEV uint32_t wc_lds_get(const Dev &d, uint32_t (*c)[64], uint32_t lane, uint32_t start, uint32_t x) {
+// Write-through variants: every state change is stored to HBM as it is made,
+// so HBM always holds each register's current state and a cache slot may be
+// overwritten without a look at what it held.  Invariant: a register held in
+// the ring is current there; one held in its LDS slot and not in the ring is
+// current there; a register leaving the ring overwrites its LDS slot.
+DEV uint32_t wt_lds_get(const Dev &d, uint32_t (*c)[64], uint32_t lane, uint32_t start, uint32_t x) {
     const uint32_t e = c[x % WALK_CACHE][lane];
     if (e && (e >> 10) == x) return e & 0x1FFu;
-    wc_evict(d, start, e);
     const uint32_t val = d.sym[start + x];
     c[x % WALK_CACHE][lane] = (x << 10) | 0x200u | val;
     return val;
 }
 
-DEV void wc_lds_put(const Dev &d, uint32_t (*c)[64], uint32_t lane, uint32_t start, uint32_t x, uint32_t val) {
-    const uint32_t e = c[x % WALK_CACHE][lane];
-    if (e && (e >> 10) != x) wc_evict(d, start, e);
-    c[x % WALK_CACHE][lane] = (x << 10) | 0x200u | val;
-}
 
 __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint32_t *gmax_all, uint64_t stride,
                                                 uint32_t copies, const Ra0Prog *progs, const uint32_t *permw,
@@ -1267,15 +1263,16 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
         if (!con || x >= size) return d.sym[v - 64u];
         const uint32_t e = wc_ring_at(ring, x & 7u);
         if (e && (e >> 10) == x) return e & 0x1FFu;
-        return wc_lds_get(d, cache, lane, start, x);
+        return wt_lds_get(d, cache, lane, start, x);
     };
     auto put = [&](uint32_t v, uint32_t val, bool own) {  // own: v is defined by the word being committed
         const uint32_t x = v - 64u - start;
-        if (!con || x >= size) { d.sym[v - 64u] = (uint16_t)val; return; }
+        d.sym[v - 64u] = (uint16_t)val;  // write-through
+        if (!con || x >= size) return;
         const uint32_t e = wc_ring_at(ring, x & 7u);
         if (e && (e >> 10) == x) { wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val); return; }
-        if (!own) { wc_lds_put(d, cache, lane, start, x, val); return; }
-        if (e) wc_lds_put(d, cache, lane, start, e >> 10, e & 0x1FFu);
+        if (!own) { cache[x % WALK_CACHE][lane] = (x << 10) | 0x200u | val; return; }
+        if (e) cache[(e >> 10) % WALK_CACHE][lane] = e;
         wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val);
     };
     for (uint32_t k = 0; k < WALK_CACHE; ++k) cache[k][lane] = 0;
@@ -1352,13 +1349,6 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
             }
         }
     }
-    if (con) {
-        for (uint32_t q = 0; q < 8; ++q) {
-            const uint32_t e = wc_ring_at(ring, q);
-            if (e) wc_lds_put(d, cache, lane, start, e >> 10, e & 0x1FFu);
-        }
-        for (uint32_t q = 0; q < WALK_CACHE; ++q) wc_evict(d, start, cache[q][lane]);
-    }
 }
 
 // gmin/gmax[e] = min/max over the copies (copy c at c * stride).
@@ -1406,15 +1396,16 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
         if (!con || x >= size) return d.sym[v - 64u];
         const uint32_t e = wc_ring_at(ring, x & 7u);
         if (e && (e >> 10) == x) return e & 0x1FFu;
-        return wc_lds_get(d, cache, lane, start, x);
+        return wt_lds_get(d, cache, lane, start, x);
     };
-    auto put = [&](uint32_t v, uint32_t val, bool own) {
+    auto put = [&](uint32_t v, uint32_t val, bool own) {  // own: v is defined by the word being committed
         const uint32_t x = v - 64u - start;
-        if (!con || x >= size) { d.sym[v - 64u] = (uint16_t)val; return; }
+        d.sym[v - 64u] = (uint16_t)val;  // write-through
+        if (!con || x >= size) return;
         const uint32_t e = wc_ring_at(ring, x & 7u);
         if (e && (e >> 10) == x) { wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val); return; }
-        if (!own) { wc_lds_put(d, cache, lane, start, x, val); return; }
-        if (e) wc_lds_put(d, cache, lane, start, e >> 10, e & 0x1FFu);
+        if (!own) { cache[x % WALK_CACHE][lane] = (x << 10) | 0x200u | val; return; }
+        if (e) cache[(e >> 10) % WALK_CACHE][lane] = e;
         wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val);
     };
     for (uint32_t q = 0; q < WALK_CACHE; ++q) cache[q][lane] = 0;
@@ -1427,11 +1418,11 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
     // as a byte store to HBM (the register bits stay)
     auto mark = [&](uint32_t v) {
         const uint32_t x = v - 64u - start;
+        sym_mark(d, v - 64u);  // write-through
         const uint32_t e = wc_ring_at(ring, x & 7u);
         if (e && (e >> 10) == x) { wc_ring_set(ring, x & 7u, e | 0x100u); return; }
         const uint32_t ce = cache[x % WALK_CACHE][lane];
-        if (ce && (ce >> 10) == x) { cache[x % WALK_CACHE][lane] = ce | 0x100u; return; }
-        sym_mark(d, v - 64u);
+        if (ce && (ce >> 10) == x) cache[x % WALK_CACHE][lane] = ce | 0x100u;
     };
     uint64_t life = 0x1Full;
     uint64_t valid = 0;  // lane 0 of a slot: region-0 entries holding a virtual register
@@ -1512,13 +1503,6 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
                 if (r.r2r != r.rdr) { Sv[slot][r.r2r] = -1; valid &= ~(1ull << r.r2r); }
             }
         }
-    }
-    if (con) {
-        for (uint32_t q = 0; q < 8; ++q) {
-            const uint32_t e = wc_ring_at(ring, q);
-            if (e) wc_lds_put(d, cache, lane, start, e >> 10, e & 0x1FFu);
-        }
-        for (uint32_t q = 0; q < WALK_CACHE; ++q) wc_evict(d, start, cache[q][lane]);
     }
 }
 
