@@ -230,10 +230,10 @@ def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: i
     log(f"{name}: {len(trees)} programs, {nodes} nodes (generated+parsed in {time.time() - t0:.1f}s)")
     ctx.upload_batch(slot, trees)
     for i in range(warmup):
-        ctx.compile_resident_batch(slot, copy_out=False)
+        ctx.compile_resident_batch(slot, copy_out=False, results=False)
     t1 = time.perf_counter()
     for i in range(steps):
-        ctx.compile_resident_batch(slot, copy_out=False)
+        ctx.compile_resident_batch(slot, copy_out=False, results=False)
     el = time.perf_counter() - t1
     ctx.profile(True)
     res = ctx.compile_resident_batch(slot, copy_out=True)

@@ -106,13 +106,15 @@ class Context:
         self._slot_programs = getattr(self, "_slot_programs", {})
         self._slot_programs[slot] = len(trees)
 
-    def compile_resident_batch(self, slot: int, copy_out: bool = True) -> list[dict]:
+    def compile_resident_batch(self, slot: int, copy_out: bool = True, results: bool = True):
+        """ptl_compile_resident_batch; one result dict per program, or (results
+        False) none: the library fills its result array either way."""
         n = getattr(self, "_slot_programs", {}).get(slot, 1)
         out = (abi.PtlResult * n)()
         s = self.lib.ptl_compile_resident_batch(self.h, slot, 1 if copy_out else 0, out)
         if s != abi.PTL_OK:
             raise PtlError(s)
-        return [result_dict(out[p], copy_out) for p in range(n)]
+        return [result_dict(out[p], copy_out) for p in range(n)] if results else None
 
     def profile(self, on: bool = True) -> None:
         self.lib.ptl_profile_enable(self.h, 1 if on else 0)
@@ -220,10 +222,14 @@ class Server:
             pass
 
 
+_EMPTY = np.zeros(0, np.uint32)
+_EMPTY.setflags(write=False)
+
+
 def result_dict(r: abi.PtlResult, copy_out: bool = True) -> dict:
     def arr(p, n):
         if not copy_out or n == 0:
-            return np.zeros(0, np.uint32)
+            return _EMPTY
         return np.ctypeslib.as_array(p, shape=(n,)).copy()
     nf = r.n_funcs
     return dict(words=arr(r.words, r.n_words), func_ids=arr(r.func_ids, nf), func_starts=arr(r.func_starts, nf),
