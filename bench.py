@@ -346,6 +346,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batches", default="batch_4096,stress_1024",
+                    help="batch workloads to measure (comma-separated; for profiling one of them)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--overlap", type=int, default=1, choices=(0, 1),
                     help="one-program allocation: walk and lockstep at the same time (1) or one after the other "
@@ -439,7 +441,7 @@ def main() -> int:
     if not args.no_batch:
         batched = [measure_batch(ctx, nm, 1 + k, args.batch_steps, 1, 8,
                                  0 if args.no_cpu_baseline or nm != "batch_4096" else 20_000_000)
-                   for k, nm in enumerate(("batch_4096", "stress_1024"))]
+                   for k, nm in enumerate(x for x in args.batches.split(",") if x)]
 
     serving = None
     if not args.no_serve:

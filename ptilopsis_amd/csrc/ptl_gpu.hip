@@ -1172,18 +1172,20 @@ constexpr uint32_t WALK_CHUNK = 32;
 // k_walk_merge folds: one table shared by every workgroup serializes its
 // atomics on the same few addresses.
 constexpr uint32_t WALK_COPIES = 16;
-// Walk and lockstep may run at the same time (one program): each walk
-// workgroup then publishes its table rows in chunks of WALK_PUB rows, adding 1
-// to done[chunk] (release, agent scope) once it has posted every row of the
-// chunk it takes part in.  The lockstep reads a row after done[chunk] reaches
-// need[chunk], the number of walk workgroups still running at the chunk's
-// first row.  Walk workgroups never wait, so when both kernels are resident
+// Walk and lockstep may run at the same time: for each program whose lockstep
+// does, each walk workgroup holding functions of it publishes the program's
+// table rows in chunks of WALK_PUB rows, adding 1 to done[cbase + chunk]
+// (release, agent scope) once it has posted every row of the chunk it takes
+// part in (the lowest lane of the program in the workgroup publishes).  The
+// lockstep reads a row after the counter reaches need[cbase + chunk], the
+// number of walk workgroups holding a function of the program that still runs
+// at the chunk's first row.  Walk workgroups never wait, so when both kernels are resident
 // the lockstep's wait always ends.  Where dispatches are serialized (a profiler
 // collecting counters), the walk cannot start before the lockstep ends: a wait
 // that sees no progress for RA0_WAIT_TICKS of the 100 MHz wall clock gives up
 // for the rest of the kernel (EF_WAIT), and the host redoes the allocation with
 // the two kernels one after the other.
-constexpr uint32_t WALK_PUB = 8;
+constexpr uint32_t WALK_PUB = 32;
 constexpr uint32_t RA0_MAXCOPIES = 8;  // table copies the lockstep folds itself
 constexpr uint64_t RA0_WAIT_TICKS = 25000000ull;  // 0.25 s
 
@@ -1194,7 +1196,10 @@ constexpr uint64_t RA0_WAIT_TICKS = 25000000ull;  // 0.25 s
 // (when some other function runs longer) one last row that gathers all later
 // steps.  Past its lane-0 functions the snapshot no longer changes and no owner
 // reads, so a later lookup only marks, whatever its step.
-struct Ra0Prog { uint32_t fbase, L, maxsize, poff; uint64_t gbase; uint32_t nf, rows; };
+// cbase: its first row chunk in the walk's publish counters (RA0_NOPUB when its
+// lockstep does not run at the same time as the walk).
+struct Ra0Prog { uint32_t fbase, L, maxsize, poff; uint64_t gbase; uint32_t nf, rows, cbase, pad; };
+constexpr uint32_t RA0_NOPUB = 0xFFFFFFFFu;
 
 // The walk's cache of a function's states, kept by one thread: ring slot x & 7
 // holds the newest definitions (tag x << 10 | valid << 9 | state); the LDS
@@ -1254,7 +1259,25 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
     const uint32_t tlast = one ? progs[pmin].rows - 1u : 0u;
     const uint64_t mbase = act ? progs[prg].gbase : 0u;  // own program's table and its last row
     const uint32_t mlast = act ? progs[prg].rows - 1u : 0u;
-    if (done && !one && lane == 0) atomicOr(d.flags, EF_OOB);  // publishing needs one program per workgroup
+    // the lowest lane of each program in the workgroup publishes that program's
+    // rows; pw = the size of the program's longest function here
+    uint32_t pw = 0;
+    bool lead = false;
+    if (done) {
+        uint64_t rem = __ballot(act);
+        while (rem) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(rem);
+            const uint32_t p = (uint32_t)__shfl((int)prg, (int)l, 64);
+            const bool mine = act && prg == p;
+            const uint64_t same = __ballot(mine);
+            uint32_t m = mine ? size : 0u;
+            for (int s = 32; s > 0; s >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, s, 64));
+            if (mine) { pw = m; lead = lane == l; }
+            rem &= ~same;
+        }
+    }
+    const uint32_t cbase = act ? progs[prg].cbase : RA0_NOPUB;
+    const bool publishes = done && lead && cbase != RA0_NOPUB;
     const bool con = size < (1u << 22);  // cache on (tags hold 22 bits of word offset)
     WcRing ring{0, 0, 0, 0, 0, 0, 0, 0};
     // state of virtual register v (inside [64, 64 + W0)); then its update
@@ -1338,14 +1361,21 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
             smin[lane] = 0xFFFFFFFFu; smax[lane] = 0;
         }
         lds_barrier();
-        if (done && one) {
-            // the workgroup leaves a chunk of rows (steps past the last row all
-            // post to it): publish it after every lane's posts are visible
-            const uint32_t r = min(i, tlast), rn = min(i + 1u, tlast);
-            if (i + 1u == wmax || r / WALK_PUB != rn / WALK_PUB) {
+        if (done) {
+            // a program's functions here leave a chunk of its rows (steps past
+            // its last row all post to that row): publish it after every
+            // lane's posts are visible
+            bool pub = false;
+            uint32_t chunk = 0;
+            if (publishes && i < pw) {
+                const uint32_t r = min(i, mlast), rn = min(i + 1u, mlast);
+                pub = i + 1u == pw || r / WALK_PUB != rn / WALK_PUB;
+                chunk = cbase + r / WALK_PUB;
+            }
+            if (__ballot(pub)) {
                 __threadfence();
                 __syncthreads();
-                if (lane == 0) __hip_atomic_fetch_add(&done[r / WALK_PUB], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (pub) __hip_atomic_fetch_add(&done[chunk], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
@@ -1793,6 +1823,9 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
     const uint32_t *perm = perm_all + pg.poff;
     const uint32_t *gmin = gmin_all + pg.gbase, *gmax = gmax_all + pg.gbase;
     const uint32_t cb = nth > 64 ? 64u : 0u;  // first thread of the table-clearing threads
+    // this program's publish counters (none: the walk has finished)
+    const uint32_t *pdone = done && pg.cbase != RA0_NOPUB ? done + pg.cbase : nullptr;
+    const uint32_t *pneed = pdone ? need + pg.cbase : nullptr;
     Ra0Fn fn[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -1822,13 +1855,13 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         if (tid < 2) keyed[1][tid] = 0;
         uint32_t m0 = 0xFFFFFFFFu, x0 = 0u;
         if (maxsize) {
-            if (done) ra0_wait(d, done, need, 0, wok);
+            if (pdone) ra0_wait(d, pdone, pneed, 0, wok);
             ra0_row(gmin, gmax, stride, copies, 0, tid, m0, x0);
         }
         Lmin[0][tid] = m0; Lmax[0][tid] = x0;
         Omin[0][tid] = 0xFFFFFFFFu; Omax[0][tid] = 0;
         if (maxsize > 1) {
-            if (done && 1u % WALK_PUB == 0) ra0_wait(d, done, need, 1u / WALK_PUB, wok);
+            if (pdone && 1u % WALK_PUB == 0) ra0_wait(d, pdone, pneed, 1u / WALK_PUB, wok);
             ra0_row(gmin, gmax, stride, copies, 1, tid, qmin, qmax);
         }
     }
@@ -2004,7 +2037,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         if (tid < 64) {
             qmin = 0xFFFFFFFFu; qmax = 0;
             if (i + 2 < maxsize) {
-                if (done && (i + 2) % WALK_PUB == 0) ra0_wait(d, done, need, (i + 2) / WALK_PUB, wok);
+                if (pdone && (i + 2) % WALK_PUB == 0) ra0_wait(d, pdone, pneed, (i + 2) / WALK_PUB, wok);
                 ra0_row(gmin, gmax, stride, copies, i + 2, tid, qmin, qmax);
             }
         }
