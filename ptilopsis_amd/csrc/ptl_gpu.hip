@@ -1029,14 +1029,21 @@ __global__ __launch_bounds__(256) void k_opt_init(Dev d) {
 // (except sources of stores).  Repeated until a sweep changes nothing.  A
 // word's sources are mostly the words just before it, so each workgroup
 // repeats its own range of OPT_RANGE words until no word of the range lost its
-// use; only chains crossing ranges need further sweeps.
+// use; only chains crossing ranges need further sweeps.  `changed` is set by a
+// clear of a word outside the clearing workgroup's range, by a word whose
+// destination lies outside its range (its flag could change under it), or by a
+// range that did not settle within its 64 rounds: a launch with none of these
+// has every range settled on values no other workgroup changed, which is the
+// fixpoint.  (Generated code names a word's own register as its destination,
+// so the second case does not arise there.)
 constexpr uint32_t OPT_RANGE = 256 * 8;
 
 __global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
     __shared__ int local;
     const uint64_t b0 = (uint64_t)blockIdx.x * OPT_RANGE;
     const uint64_t b1 = b0 + OPT_RANGE < d.W0 ? b0 + OPT_RANGE : d.W0;
-    bool any = false;
+    bool cross = false;  // a clear outside the range, or a destination outside it
+    bool settled = false;
     for (int round = 0; round < 64; ++round) {
         if (threadIdx.x == 0) local = 0;
         __syncthreads();
@@ -1045,16 +1052,26 @@ __global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
             const uint32_t rd = d.w0_rd[i];
             if (!((int32_t)rd > 63)) continue;
             const uint32_t v = rd - 64u;
-            if (v >= d.W0 || d.used[v]) continue;
+            if (v >= d.W0) continue;
+            cross |= (uint64_t)v - b0 >= b1 - b0;
+            if (d.used[v]) continue;
             const int32_t a = (int32_t)(d.w0_rs1[i] - 64u), b = (int32_t)(d.w0_rs2[i] - 64u);
-            if (a >= 0 && (uint32_t)a < d.W0 && !d.sstore[a] && d.used[a]) { d.used[a] = 0; any = true; loc |= (uint64_t)a >= b0 && (uint64_t)a < b1; }
-            if (b >= 0 && (uint32_t)b < d.W0 && !d.sstore[b] && d.used[b]) { d.used[b] = 0; any = true; loc |= (uint64_t)b >= b0 && (uint64_t)b < b1; }
+            if (a >= 0 && (uint32_t)a < d.W0 && !d.sstore[a] && d.used[a]) {
+                d.used[a] = 0;
+                const bool in = (uint64_t)a >= b0 && (uint64_t)a < b1;
+                loc |= in; cross |= !in;
+            }
+            if (b >= 0 && (uint32_t)b < d.W0 && !d.sstore[b] && d.used[b]) {
+                d.used[b] = 0;
+                const bool in = (uint64_t)b >= b0 && (uint64_t)b < b1;
+                loc |= in; cross |= !in;
+            }
         }
         if (loc) local = 1;
         __syncthreads();
-        if (!local) break;
+        if (!local) { settled = true; break; }
     }
-    if (any) *changed = 1;
+    if (cross || (!settled && threadIdx.x == 0)) *changed = 1;
 }
 
 __global__ __launch_bounds__(256) void k_opt_final(Dev d) {
