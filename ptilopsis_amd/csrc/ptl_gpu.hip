@@ -2160,9 +2160,25 @@ DEV uint32_t spill_of(const Dev &d, uint32_t v) {
     return sym_swapped(d, d.fstarts[lo]) ? 1u : 0u;
 }
 
+// The function holding word w (last start <= w) and its word range [fs, fe).
+DEV void func_range(const Dev &d, uint32_t w, uint32_t &fs, uint32_t &fe) {
+    uint32_t lo = 0, hi = d.F;
+    while (hi - lo > 1) { uint32_t m = (lo + hi) / 2; if (d.fstarts[m] <= w) lo = m; else hi = m; }
+    fs = d.fstarts[lo];
+    fe = lo + 1 < d.F ? d.fstarts[lo + 1] : 0xFFFFFFFFu;
+}
+
 // Emit (:1552-1697).
 __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    // the function of the wave's first word, looked up once for the wave: a
+    // word's operands are mostly registers of its own function, and the
+    // wave's words mostly of one function (spill_of of any other register)
+    const uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~63u);
+    uint32_t fs = 0, fe = 0;
+    if (i0 < d.W0 && d.F) func_range(d, (uint32_t)i0, fs, fe);
+    const uint32_t fsp = fe > fs && sym_swapped(d, fs) ? 1u : 0u;
+    auto spill = [&](uint32_t v) -> uint32_t { return v >= fs && v < fe ? fsp : spill_of(d, v); };
     if (i >= d.W0 || !d.usedi[i]) return;
     const int32_t base = (int32_t)d.ra_off[i];
     const uint32_t ins = d.w0_instr[i];
@@ -2175,14 +2191,14 @@ __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
         const uint32_t e = d.sym[r1 - 64u];
         const bool sw = (e >> 8) != 0;
         l1 = sw ? base : -1;
-        st1 = spill_of(d, r1 - 64u);
+        st1 = spill(r1 - 64u);
         a1 = sw ? (((ins == 0xC0007053u) || (ins & 0x7Fu) == 0x53u) ? 37u : 5u) : (e & 0xFFu);
     }
     if ((int32_t)r2 > 63) {
         const uint32_t e = d.sym[r2 - 64u];
         const bool sw = (e >> 8) != 0;
         l2 = sw ? base : -1;  // the reference tests its own, still -1, offset (D6)
-        st2 = spill_of(d, r2 - 64u);
+        st2 = spill(r2 - 64u);
         a2 = sw ? (((ins & 0x7Fu) == 0x53u) ? 38u : 6u) : (e & 0xFFu);
     }
     const int32_t mo = base + (l1 > 0 ? 1 : 0) + (l2 > 0 ? 1 : 0);
@@ -2190,7 +2206,7 @@ __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
         const uint32_t e = d.sym[rdv - 64u];
         const bool sw = (e >> 8) != 0;
         rdo = sw ? mo + 1 : -1;
-        std_ = spill_of(d, rdv - 64u);
+        std_ = spill(rdv - 64u);
         ad = e & 0xFFu;
     }
     if ((rdo >= 0 && (uint32_t)rdo >= d.W1) || (uint32_t)mo >= d.W1) { atomicOr(d.flags, EF_OOB); return; }
