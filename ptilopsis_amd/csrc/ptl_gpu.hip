@@ -751,6 +751,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_down(const Seg *segs, ui
                                                              const uint2 *tile_pre, uint32_t *Lpos, uint32_t *Rpos,
                                                              uint2 *base) {
     __shared__ uint32_t sL[SCAN_BLOCK / 64], sR[SCAN_BLOCK / 64];
+    // the tile's stops, staged: its left and right stops are each one
+    // contiguous range of Lpos / Rpos, written out coalesced at the end
+    __shared__ uint32_t stL[SCAN_TILE], stR[SCAN_TILE];
     const uint32_t a0 = blockIdx.x * (uint32_t)SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t bits = 0, cl = 0, cr = 0;  // 2 bits per item
     if (a0 < A) {
@@ -771,7 +774,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_down(const Seg *segs, ui
     const uint32_t xl = wave_incl_scan(cl, lane), xr = wave_incl_scan(cr, lane);
     if (lane == 63) { sL[wid] = xl; sR[wid] = xr; }
     __syncthreads();
-    uint32_t rl = tile_pre[blockIdx.x].x + xl - cl, rr = tile_pre[blockIdx.x].y + xr - cr;
+    const uint32_t tL = tile_pre[blockIdx.x].x, tR = tile_pre[blockIdx.x].y;
+    uint32_t rl = xl - cl, rr = xr - cr;  // offsets within the tile's ranges
 #pragma unroll
     for (int w = 0; w < SCAN_BLOCK / 64; ++w) if ((uint32_t)w < wid) { rl += sL[w]; rr += sR[w]; }
     if (a0 < A) {
@@ -781,12 +785,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_down(const Seg *segs, ui
             if (a >= A) break;
             w.to(a);
             const uint32_t i = w.pos(a);
-            if (a == w.g.off) base[w.s] = make_uint2(rl, rr);
+            if (a == w.g.off) base[w.s] = make_uint2(tL + rl, tR + rr);
             const uint32_t b = (bits >> (2 * k)) & 3u;
-            if (b & 1u) Lpos[rl++] = i;
-            if (b & 2u) Rpos[rr++] = i;
+            if (b & 1u) stL[rl++] = i;
+            if (b & 2u) stR[rr++] = i;
         }
     }
+    uint32_t nL = 0, nR = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) { nL += sL[w]; nR += sR[w]; }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nL; k += SCAN_BLOCK) Lpos[tL + k] = stL[k];
+    for (uint32_t k = threadIdx.x; k < nR; k += SCAN_BLOCK) Rpos[tR + k] = stR[k];
 }
 
 struct SegCut { uint32_t baseL, cntL, baseR, cntR, m, cut, uni, pad; };
