@@ -1193,6 +1193,7 @@ DEV void ra_commit_syms(const Dev &d, const RaStep &r) {
 // into LDS WALK_CHUNK steps at a time, loaded cooperatively (consecutive
 // threads, consecutive words of one function).
 constexpr uint32_t WALK_CACHE = 64;
+constexpr uint32_t WALK_RING = 3;  // register ring of the newest definitions: 4 entries (index mask; 2, 8 and 16 were slower)
 constexpr uint32_t WALK_CHUNK = 32;
 // Workgroups spread their per-step lookup records over up to WALK_COPIES
 // copies of the (step, register) tables (copy = workgroup mod copies), which
@@ -1228,11 +1229,12 @@ constexpr uint64_t RA0_WAIT_TICKS = 25000000ull;  // 0.25 s
 struct Ra0Prog { uint32_t fbase, L, maxsize, poff; uint64_t gbase; uint32_t nf, rows, cbase, pad; };
 constexpr uint32_t RA0_NOPUB = 0xFFFFFFFFu;
 
-// The walk's cache of a function's states, kept by one thread: ring slot x & 7
-// holds the newest definitions (tag x << 10 | valid << 9 | state); the LDS
-// cache c[x % WALK_CACHE][lane] holds older ones; HBM (sym) the rest.
-// The ring is eight named registers (an array would be indexed through
-// scratch memory).
+// The walk's cache of a function's states, kept by one thread: ring slot
+// x & WALK_RING holds the newest definitions (tag x << 10 | valid << 9 |
+// state); the LDS cache c[x % WALK_CACHE][lane] holds older ones; HBM (sym)
+// all of them (written through).  The ring is named registers selected by
+// index bits (an array would be indexed through scratch memory); the walks use
+// four of its eight.
 struct WcRing { uint32_t r0, r1, r2, r3, r4, r5, r6, r7; };
 
 DEV uint32_t wc_ring_at(const WcRing &g, uint32_t k) {
@@ -1311,7 +1313,7 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
     auto get = [&](uint32_t v) -> uint32_t {
         const uint32_t x = v - 64u - start;
         if (!con || x >= size) return d.sym[v - 64u];
-        const uint32_t e = wc_ring_at(ring, x & 7u);
+        const uint32_t e = wc_ring_at(ring, x & WALK_RING);
         if (e && (e >> 10) == x) return e & 0x1FFu;
         return wt_lds_get(d, cache, lane, start, x);
     };
@@ -1319,11 +1321,11 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
         const uint32_t x = v - 64u - start;
         d.sym[v - 64u] = (uint16_t)val;  // write-through
         if (!con || x >= size) return;
-        const uint32_t e = wc_ring_at(ring, x & 7u);
-        if (e && (e >> 10) == x) { wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val); return; }
+        const uint32_t e = wc_ring_at(ring, x & WALK_RING);
+        if (e && (e >> 10) == x) { wc_ring_set(ring, x & WALK_RING, (x << 10) | 0x200u | val); return; }
         if (!own) { cache[x % WALK_CACHE][lane] = (x << 10) | 0x200u | val; return; }
         if (e) cache[(e >> 10) % WALK_CACHE][lane] = e;
-        wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val);
+        wc_ring_set(ring, x & WALK_RING, (x << 10) | 0x200u | val);
     };
     for (uint32_t k = 0; k < WALK_CACHE; ++k) cache[k][lane] = 0;
     fstart[lane] = start; fsize[lane] = size;
@@ -1451,7 +1453,7 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
     auto get = [&](uint32_t v) -> uint32_t {
         const uint32_t x = v - 64u - start;
         if (!con || x >= size) return d.sym[v - 64u];
-        const uint32_t e = wc_ring_at(ring, x & 7u);
+        const uint32_t e = wc_ring_at(ring, x & WALK_RING);
         if (e && (e >> 10) == x) return e & 0x1FFu;
         return wt_lds_get(d, cache, lane, start, x);
     };
@@ -1459,11 +1461,11 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
         const uint32_t x = v - 64u - start;
         d.sym[v - 64u] = (uint16_t)val;  // write-through
         if (!con || x >= size) return;
-        const uint32_t e = wc_ring_at(ring, x & 7u);
-        if (e && (e >> 10) == x) { wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val); return; }
+        const uint32_t e = wc_ring_at(ring, x & WALK_RING);
+        if (e && (e >> 10) == x) { wc_ring_set(ring, x & WALK_RING, (x << 10) | 0x200u | val); return; }
         if (!own) { cache[x % WALK_CACHE][lane] = (x << 10) | 0x200u | val; return; }
         if (e) cache[(e >> 10) % WALK_CACHE][lane] = e;
-        wc_ring_set(ring, x & 7u, (x << 10) | 0x200u | val);
+        wc_ring_set(ring, x & WALK_RING, (x << 10) | 0x200u | val);
     };
     for (uint32_t q = 0; q < WALK_CACHE; ++q) cache[q][lane] = 0;
     fstart[lane] = start; fsize[lane] = size;
@@ -1476,8 +1478,8 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
     auto mark = [&](uint32_t v) {
         const uint32_t x = v - 64u - start;
         sym_mark(d, v - 64u);  // write-through
-        const uint32_t e = wc_ring_at(ring, x & 7u);
-        if (e && (e >> 10) == x) { wc_ring_set(ring, x & 7u, e | 0x100u); return; }
+        const uint32_t e = wc_ring_at(ring, x & WALK_RING);
+        if (e && (e >> 10) == x) { wc_ring_set(ring, x & WALK_RING, e | 0x100u); return; }
         const uint32_t ce = cache[x % WALK_CACHE][lane];
         if (ce && (ce >> 10) == x) cache[x % WALK_CACHE][lane] = ce | 0x100u;
     };
