@@ -158,13 +158,26 @@ int64_t ptl_debug_dump(ptl_ctx *ctx, const char *name, uint32_t *dst, uint64_t c
  * ptl_compile (0 = all stages); for stage-boundary parity tests. */
 int ptl_debug_stop_after(ptl_ctx *ctx, int stage);
 
-/* Register allocation of a compile whose only multi-group program goes through
- * the walk and the lockstep runs the two kernels at the same time (on = 1, the
- * default) or one after the other (on = 0); the output is the same.  on = 2
- * runs them at the same time but has the lockstep wait for one more walk
- * workgroup than there is: its bounded wait gives up (0.25 s) and the library
- * redoes the allocation one kernel after the other, as it does when the device
- * serializes the two kernels.  PTL_ERR_ARG for other values. */
+/* Launch parameters chosen by the autotuner (ptilopsis_amd/tune.py writes
+ * ptilopsis_amd/tuning.json; the Python Context applies it): per size bucket of
+ * a compile's node count (0: < 64Ki nodes, 1: < 1Mi, 2: < 8Mi, 3: larger), the
+ * block size of each stage's per-node and per-word kernels ("block.preprocess",
+ * "block.isn_cnt", "block.isn_gen", "block.optimize", "block.emit",
+ * "block.fix_jumps": 64, 128 or 256) and the register-allocation walks' ring
+ * ("walk_ring": 4 or 8 entries).  Output never depends on them.  No counterpart
+ * in the reference.  PTL_ERR_ARG for unknown knobs, buckets or values. */
+/* With ctx NULL, ptl_tune_set / ptl_tune_get address the process-wide defaults
+ * that each context (the serving workers' included) copies at creation. */
+int ptl_tune_set(ptl_ctx *ctx, const char *knob, uint32_t bucket, uint32_t value);
+int ptl_tune_get(ptl_ctx *ctx, const char *knob, uint32_t bucket, uint32_t *value);
+
+/* Register allocation: lockstep launches of few programs run at the same time
+ * as the walk (on = 1, the default) or every lockstep after it (on = 0); the
+ * output is the same.  on = 2 overlaps but has the first overlapped program's
+ * lockstep wait for one more walk workgroup than there is: its bounded wait
+ * gives up (0.25 s) and the library redoes the allocation one kernel after the
+ * other, as it does when the device serializes the kernels.  PTL_ERR_ARG for
+ * other values. */
 int ptl_debug_overlap(ptl_ctx *ctx, int on);
 
 /* Per-kernel timing: with profiling on, every kernel launch of the following

@@ -16,7 +16,7 @@ EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_uplo
            "ptl_result_device", "ptl_compile_batch", "ptl_upload_batch", "ptl_compile_resident_batch",
            "ptl_server_create", "ptl_server_destroy", "ptl_server_submit", "ptl_server_wait",
            "ptl_server_release", "ptl_server_stats",
-           "ptl_result_asm", "ptl_disassemble",
+           "ptl_result_asm", "ptl_disassemble", "ptl_tune_set", "ptl_tune_get",
            "ptl_debug_dump", "ptl_debug_stop_after", "ptl_debug_overlap",
            "ptl_debug_exclusive_scan", "ptl_profile_enable", "ptl_profile_get")
 
@@ -85,6 +85,10 @@ def bind(lib):
     lib.ptl_result_asm.restype = C.c_int
     lib.ptl_disassemble.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
     lib.ptl_disassemble.restype = C.c_int
+    lib.ptl_tune_set.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32]
+    lib.ptl_tune_set.restype = C.c_int
+    lib.ptl_tune_get.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32, C.POINTER(C.c_uint32)]
+    lib.ptl_tune_get.restype = C.c_int
     lib.ptl_debug_overlap.argtypes = [C.c_void_p, C.c_int]
     lib.ptl_debug_overlap.restype = C.c_int
     lib.ptl_debug_exclusive_scan.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]

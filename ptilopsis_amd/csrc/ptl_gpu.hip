@@ -44,6 +44,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "rv_spec.h"
 #include "../../include/ptl.h"
@@ -1193,7 +1194,9 @@ DEV void ra_commit_syms(const Dev &d, const RaStep &r) {
 // into LDS WALK_CHUNK steps at a time, loaded cooperatively (consecutive
 // threads, consecutive words of one function).
 constexpr uint32_t WALK_CACHE = 64;
-constexpr uint32_t WALK_RING = 3;  // register ring of the newest definitions: 4 entries (index mask; 2, 8 and 16 were slower)
+// The walks' register ring of the newest definitions holds RING + 1 entries
+// (RING is the index mask: 3 or 7, chosen by the autotuner per size bucket;
+// 2 and 16 entries measured slower on every workload).
 constexpr uint32_t WALK_CHUNK = 32;
 // Workgroups spread their per-step lookup records over up to WALK_COPIES
 // copies of the (step, register) tables (copy = workgroup mod copies), which
@@ -1263,6 +1266,7 @@ DEV uint32_t wt_lds_get(const Dev &d, uint32_t (*c)[64], uint32_t lane, uint32_t
 }
 
 
+template <uint32_t WALK_RING>
 __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint32_t *gmax_all, uint64_t stride,
                                                 uint32_t copies, const Ra0Prog *progs, const uint32_t *permw,
                                                 uint32_t nw, uint32_t *done) {
@@ -1433,6 +1437,7 @@ __global__ void k_walk_merge(uint32_t *gmin, uint32_t *gmax, uint64_t stride, ui
 // operand read of the step, as in the vector loop.  A mark on a register outside
 // lane 0's own function cannot be kept in its cache: the program then takes the
 // exact serial path.
+template <uint32_t WALK_RING>
 __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, uint32_t np) {
     __shared__ uint32_t cache[WALK_CACHE][64];
     __shared__ uint4 words[WALK_CHUNK][64];

@@ -15,7 +15,7 @@ import sys
 
 import numpy as np
 
-from . import abi
+from . import abi, tune
 from .native import Tree, hip_lib
 
 
@@ -30,6 +30,7 @@ class Context:
 
     def __init__(self, device: int = 0, arena_bytes: int = 0):
         self.lib = hip_lib()
+        tune.apply_defaults(self.lib)
         h = C.c_void_p()
         s = self.lib.ptl_create(C.byref(h), device, arena_bytes)
         if s != abi.PTL_OK:
@@ -147,6 +148,19 @@ class Context:
         t, n = C.c_void_p(), C.c_uint64()
         return self._text(self.lib.ptl_disassemble(self.h, w.ctypes.data, w.shape[0], C.byref(t), C.byref(n)), t, n)
 
+    def tune_set(self, knob: str, bucket: int, value: int) -> None:
+        """One launch parameter of this context (include/ptl.h, ptl_tune_set)."""
+        s = self.lib.ptl_tune_set(self.h, knob.encode(), bucket, value)
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+
+    def tune_get(self, knob: str, bucket: int) -> int:
+        v = C.c_uint32()
+        s = self.lib.ptl_tune_get(self.h, knob.encode(), bucket, C.byref(v))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return int(v.value)
+
     def set_overlap(self, on) -> None:
         """Walk and lockstep at the same time (True, default) or one after the
         other (False); 2 forces the lockstep's bounded wait to give up and the
@@ -176,6 +190,7 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, max_batch: int = 4096, workers: int = 2):
         self.lib = hip_lib()
+        tune.apply_defaults(self.lib)
         h = C.c_void_p()
         s = self.lib.ptl_server_create(C.byref(h), device, hbm_budget, max_batch, workers)
         if s != abi.PTL_OK:

@@ -165,6 +165,30 @@ def test_gpu_walk_and_lockstep_overlap(gpu_ctx, which):
         assert_same(got[on], want)
 
 
+@pytest.mark.parametrize("which", ["single_50k", "stress"])
+def test_gpu_tuning_knobs_do_not_change_output(gpu_ctx, which):
+    # every launch parameter the autotuner may pick gives the same words
+    t = W.single_50k() if which == "single_50k" else W.program_of_size((W.SEED_STRESS << 32) + 5, 64000, W.STRESS)
+    want = oracle.compile_tree(t)
+    b = 0 if t.n < (64 << 10) else 1
+    knobs = ("block.preprocess", "block.isn_cnt", "block.isn_gen", "block.optimize", "block.emit", "block.fix_jumps")
+    old = {k: gpu_ctx.tune_get(k, b) for k in knobs + ("walk_ring",)}
+    try:
+        for bs, ring in ((64, 8), (128, 4), (64, 4)):
+            say("tuning", which, bs, ring)
+            for k in knobs:
+                gpu_ctx.tune_set(k, b, bs)
+            gpu_ctx.tune_set("walk_ring", b, ring)
+            assert_same(gpu_ctx.compile(t), want)
+    finally:
+        for k, v in old.items():
+            gpu_ctx.tune_set(k, b, v)
+    with pytest.raises(PtlError):
+        gpu_ctx.tune_set("block.emit", 0, 96)
+    with pytest.raises(PtlError):
+        gpu_ctx.tune_set("walk_ring", 4, 4)
+
+
 def test_gpu_many_small_functions(gpu_ctx):
     src, _ = W.program_source(31, 300, min_stmts=1, max_stmts=3)
     t = parse_source(src)
