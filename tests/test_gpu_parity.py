@@ -189,6 +189,23 @@ def test_gpu_tuning_knobs_do_not_change_output(gpu_ctx, which):
         gpu_ctx.tune_set("walk_ring", 4, 4)
 
 
+@pytest.mark.parametrize("nf", [40000, 66000])
+def test_gpu_lockstep_largest_classes(gpu_ctx, nf):
+    # 40000 functions: 5000 groups, the 8192-group class (k_ra_lane0<8, 8192>);
+    # 66000: 8250 groups, past it (k_ra_lane0_generic)
+    src, _ = W.program_source(77, nf, min_stmts=1, max_stmts=1)
+    t = parse_source(src)
+    say("lockstep class of", nf, "functions", t.n)
+    gpu_ctx.profile(True)
+    try:
+        r = gpu_ctx.compile(t)
+        names = gpu_ctx.kernel_times()
+    finally:
+        gpu_ctx.profile(False)
+    assert ("k_ra_lane0_generic" in names) == (nf > 65536)
+    assert_same(r, oracle.compile_tree(t))
+
+
 def test_gpu_many_small_functions(gpu_ctx):
     src, _ = W.program_source(31, 300, min_stmts=1, max_stmts=3)
     t = parse_source(src)
