@@ -669,10 +669,37 @@ DEV uint32_t seg_find(const Seg *segs, uint32_t m, uint32_t a) {
 }
 
 // Walks SCAN_ITEMS consecutive active indices from a0, tracking their range.
+// Last range whose active offset is <= a, among ranges [lo, hi) (the answer
+// is known to lie there).
+DEV uint32_t seg_find_in(const Seg *segs, uint32_t lo, uint32_t hi, uint32_t a) {
+    while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (segs[mid].off <= a) lo = mid; else hi = mid; }
+    return lo;
+}
+
+// The ranges holding a workgroup's items [a_lo, a_hi]: one search per
+// workgroup, so each thread then searches only this window.
+// Called by every thread of the workgroup; items [a_lo, a_lo + span) clipped to A.
+struct SegWindow { uint32_t lo, hi; };
+DEV SegWindow seg_window(const Seg *segs, uint32_t m, uint32_t A, uint32_t a_lo, uint32_t span) {
+    __shared__ uint32_t w[2];
+    if (a_lo < A) {
+        const uint32_t a_hi = min(A, a_lo + span) - 1;
+        if (threadIdx.x == 0) w[0] = seg_find(segs, m, a_lo);
+        if (threadIdx.x == 64) w[1] = seg_find(segs, m, a_hi);
+    }
+    __syncthreads();
+    const SegWindow r{w[0], w[1] + 1};
+    __syncthreads();
+    return r;
+}
+
 struct SegWalk {
     const Seg *segs; uint32_t m, s, nxt; Seg g;
     DEV SegWalk(const Seg *sg, uint32_t mm, uint32_t a0) : segs(sg), m(mm) {
         s = seg_find(sg, mm, a0); g = sg[s]; nxt = s + 1 < mm ? sg[s + 1].off : 0xFFFFFFFFu;
+    }
+    DEV SegWalk(const Seg *sg, uint32_t mm, SegWindow wd, uint32_t a0) : segs(sg), m(mm) {
+        s = seg_find_in(sg, wd.lo, wd.hi, a0); g = sg[s]; nxt = s + 1 < mm ? sg[s + 1].off : 0xFFFFFFFFu;
     }
     DEV void to(uint32_t a) {
         while (a >= nxt) { ++s; g = segs[s]; nxt = s + 1 < m ? segs[s + 1].off : 0xFFFFFFFFu; }
@@ -693,10 +720,11 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_reduce(const Seg *segs, 
                                                                const int32_t *pivot, const int32_t *skey,
                                                                uint2 *tile_sums) {
     __shared__ uint32_t sL[SCAN_BLOCK / 64], sR[SCAN_BLOCK / 64];
+    const SegWindow wd = seg_window(segs, m, A, blockIdx.x * (uint32_t)SCAN_TILE, (uint32_t)SCAN_TILE);
     const uint32_t a0 = blockIdx.x * (uint32_t)SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t cl = 0, cr = 0;
     if (a0 < A) {
-        SegWalk w(segs, m, a0);
+        SegWalk w(segs, m, wd, a0);
         int32_t p = pivot[w.s];
         for (int k = 0; k < SCAN_ITEMS; ++k) {
             const uint32_t a = a0 + k;
@@ -755,10 +783,11 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_down(const Seg *segs, ui
     // the tile's stops, staged: its left and right stops are each one
     // contiguous range of Lpos / Rpos, written out coalesced at the end
     __shared__ uint32_t stL[SCAN_TILE], stR[SCAN_TILE];
+    const SegWindow wd = seg_window(segs, m, A, blockIdx.x * (uint32_t)SCAN_TILE, (uint32_t)SCAN_TILE);
     const uint32_t a0 = blockIdx.x * (uint32_t)SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t bits = 0, cl = 0, cr = 0;  // 2 bits per item
     if (a0 < A) {
-        SegWalk w(segs, m, a0);
+        SegWalk w(segs, m, wd, a0);
         int32_t p = pivot[w.s];
         for (int k = 0; k < SCAN_ITEMS; ++k) {
             const uint32_t a = a0 + k;
@@ -780,7 +809,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_down(const Seg *segs, ui
 #pragma unroll
     for (int w = 0; w < SCAN_BLOCK / 64; ++w) if ((uint32_t)w < wid) { rl += sL[w]; rr += sR[w]; }
     if (a0 < A) {
-        SegWalk w(segs, m, a0);
+        SegWalk w(segs, m, wd, a0);
         for (int k = 0; k < SCAN_ITEMS; ++k) {
             const uint32_t a = a0 + k;
             if (a >= A) break;
@@ -866,9 +895,10 @@ __global__ void k_sort_search(const Seg *segs, uint32_t m, const uint2 *base, co
 __global__ __launch_bounds__(256) void k_sort_swap(const Seg *segs, uint32_t m, uint32_t A, const SegCut *cuts,
                                                    const uint32_t *Lpos, const uint32_t *Rpos, uint32_t *sidx,
                                                    int32_t *skey) {
+    const SegWindow wd = seg_window(segs, m, A, blockIdx.x * 256u * SCAN_ITEMS, 256u * SCAN_ITEMS);
     const uint32_t a0 = blockIdx.x * 256u * SCAN_ITEMS + threadIdx.x;
     if (a0 >= A) return;
-    SegWalk w(segs, m, a0);
+    SegWalk w(segs, m, wd, a0);
     SegCut c = cuts[w.s];
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         const uint32_t a = a0 + 256u * k;
