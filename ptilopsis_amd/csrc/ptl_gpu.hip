@@ -1865,6 +1865,24 @@ DEV void ra0_wait(const Dev &d, const uint32_t *done, const uint32_t *need, uint
     }
 }
 
+// Stages row r of the walk's tables in LDS (smn/smx[copy]) for wave 0 to fold
+// a step later: copy k is loaded by wave nwv-1-(k mod nwv), the last waves
+// holding the shortest functions (idle through most steps), so that no wave
+// with work waits for rows coming from HBM.  A loading wave waits for the
+// row's chunk to be published (`first`: for the chunk of a first row).
+DEV void ra0_stage(const Dev &d, const uint32_t *gmin, const uint32_t *gmax, uint64_t stride, uint32_t copies,
+                   uint32_t r, const uint32_t *pdone, const uint32_t *pneed, bool first, bool &ok,
+                   uint32_t (*smn)[64], uint32_t (*smx)[64], uint32_t wid, uint32_t nwv, uint32_t lane) {
+    const uint32_t k0 = nwv - 1u - wid;
+    if (k0 >= copies) return;
+    if (pdone && (first || r % WALK_PUB == 0)) ra0_wait(d, pdone, pneed, r / WALK_PUB, ok);
+    for (uint32_t k = k0; k < copies; k += nwv) {
+        const uint64_t o = (uint64_t)k * stride + (uint64_t)r * 64u + lane;
+        smn[k][lane] = gmin[o];
+        smx[k][lane] = gmax[o];
+    }
+}
+
 template <int PER, uint32_t NL>
 __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, const uint32_t *gmin_all,
                                                    const uint32_t *gmax_all, const uint32_t *perm_all,
@@ -1879,8 +1897,10 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
     __shared__ uint32_t flip[64];                             // flip bits per snapshot entry
     __shared__ uint32_t mk[2][2];                             // snapshot entries marked at step i
     __shared__ uint32_t keyed[2][2];                          // snapshot entries rewritten in the commit of step i
+    __shared__ uint32_t smn[RA0_MAXCOPIES][64], smx[RA0_MAXCOPIES][64];  // staged row of step i+1, per copy
     __shared__ int changed;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    const uint32_t wid = tid >> 6, nwv = nth >> 6, lane = tid & 63u;
     // this workgroup's program: groups of 8 functions from fbase, L of them
     const Ra0Prog pg = progs[blockIdx.x];
     const uint32_t L = pg.L, maxsize = pg.rows, fbase = pg.fbase;  // steps: the table's rows
@@ -1912,8 +1932,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             cur_rd[f.j] = act ? f.ord : 0u; cur_rs1[f.j] = act ? f.rs1 : 0u; cur_rs2[f.j] = act ? f.rs2 : 0u;
         }
     }
-    uint32_t qmin = 0xFFFFFFFFu, qmax = 0;  // wave 0: group lookups of step i+1
-    bool wok = true;                         // wave 0: the walk's rows still arrive
+    bool wok = true;  // the walk's rows still arrive (waves that wait for them)
     if (tid < 64) {
         Sv[0][tid] = -1; So[0][tid] = -1; wkey[0][tid] = 0;
         if (tid < 2) keyed[1][tid] = 0;
@@ -1923,20 +1942,17 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             ra0_row(gmin, gmax, stride, copies, 0, tid, m0, x0);
         }
         Lmin[0][tid] = m0; Lmax[0][tid] = x0;
-        Omin[0][tid] = 0xFFFFFFFFu; Omax[0][tid] = 0;
-        if (maxsize > 1) {
-            if (pdone && 1u % WALK_PUB == 0) ra0_wait(d, pdone, pneed, 1u / WALK_PUB, wok);
-            ra0_row(gmin, gmax, stride, copies, 1, tid, qmin, qmax);
-        }
+        Omin[0][tid] = Omin[1][tid] = 0xFFFFFFFFu; Omax[0][tid] = Omax[1][tid] = 0;
     }
+    if (maxsize > 1) ra0_stage(d, gmin, gmax, stride, copies, 1, pdone, pneed, true, wok, smn, smx, wid, nwv, lane);
     __syncthreads();
     uint32_t fb = 0;   // wave 0: flip bits of snapshot register tid
+    uint32_t om = 0xFFFFFFFFu, ox = 0;  // wave 0: owners' lookups of register tid at step i
     for (uint32_t i = 0; i < maxsize; ++i) {
         const uint32_t c = i & 1u, n = c ^ 1u;
         // P1: every group evaluates with no flips; a group that cannot own a
         // snapshot entry finalizes its step, a possible owner keeps its
         // lookups apart
-        if (tid < 64) { Lmin[n][tid] = qmin; Lmax[n][tid] = qmax; Omin[n][tid] = 0xFFFFFFFFu; Omax[n][tid] = 0; }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             Ra0Fn &f = fn[q];
@@ -1959,9 +1975,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         // P2: an owner of a snapshot entry reading it as an operand sees the
         // swap when a lower group looked the register up in this step
         if (tid < 64) {
+            om = Omin[c][tid]; ox = Omax[c][tid];
+            Omin[c][tid] = 0xFFFFFFFFu; Omax[c][tid] = 0;  // cleared for the settle rounds and step i+2
             uint32_t nb = 0;
             const int32_t o = So[c][tid], v = Sv[c][tid];
-            if (v >= 64 && o >= 0 && min(Lmin[c][tid], Omin[c][tid]) < (uint32_t)o) {
+            if (v >= 64 && o >= 0 && min(Lmin[c][tid], om) < (uint32_t)o) {
                 if (cur_rs1[o] == (uint32_t)v) nb |= 1u;
                 if (cur_rs2[o] == (uint32_t)v) nb |= 2u;
             }
@@ -1969,6 +1987,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             fb = nb;
             const uint64_t any = __ballot(nb != 0);
             if (tid == 0) changed = any != 0;
+            // the walk's lookups of step i+1 (staged at the end of step i-1)
+            uint32_t mn = 0xFFFFFFFFu, mx = 0;
+            if (i + 1 < maxsize)
+                for (uint32_t k = 0; k < copies; ++k) { mn = min(mn, smn[k][tid]); mx = max(mx, smx[k][tid]); }
+            Lmin[n][tid] = mn; Lmax[n][tid] = mx;
         }
         if (tid >= cb)
             for (uint32_t k = tid - cb; k < 64; k += nth - cb) wkey[n][k] = 0;
@@ -1976,28 +1999,25 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         if (changed) {
             // settle intra-step flips among the owners: re-evaluate until the
             // flip bits are stable (lower groups settle first, so this reaches
-            // the in-order result)
+            // the in-order result); O[c] was cleared when wave 0 last read it
             for (uint32_t round = 0;; ++round) {
-                if (tid < 64) { Omin[c][tid] = 0xFFFFFFFFu; Omax[c][tid] = 0; }
-#pragma unroll
-                for (int q = 0; q < PER; ++q)
-                    if (fn[q].own && (int32_t)i < (int32_t)fn[q].size)
-                        fn[q].pk = (fn[q].pk & 0x3FFFFu) |
-                                   (ra0_flips(ra0_owned(fn[q].own, Sv[c], So[c], fn[q].j), flip) << 18);
-                lds_barrier();
 #pragma unroll
                 for (int q = 0; q < PER; ++q) {
                     const uint32_t j = fn[q].j;
                     if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
+                        fn[q].pk = (fn[q].pk & 0x3FFFFu) |
+                                   (ra0_flips(ra0_owned(fn[q].own, Sv[c], So[c], j), flip) << 18);
                         ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
                         ra0_lookups(Omin[c], Omax[c], ra0_step(fn[q]), j);
                     }
                 }
                 lds_barrier();
                 if (tid < 64) {
+                    om = Omin[c][tid]; ox = Omax[c][tid];
+                    Omin[c][tid] = 0xFFFFFFFFu; Omax[c][tid] = 0;
                     uint32_t nb = 0;
                     const int32_t o = So[c][tid], v = Sv[c][tid];
-                    if (v >= 64 && o >= 0 && min(Lmin[c][tid], Omin[c][tid]) < (uint32_t)o) {
+                    if (v >= 64 && o >= 0 && min(Lmin[c][tid], om) < (uint32_t)o) {
                         if (cur_rs1[o] == (uint32_t)v) nb |= 1u;
                         if (cur_rs2[o] == (uint32_t)v) nb |= 2u;
                     }
@@ -2027,7 +2047,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         if (tid < 64) {
             const int32_t v = Sv[c][tid], o = So[c][tid];
             Sv[n][tid] = v; So[n][tid] = o;
-            const uint32_t mx = max(Lmax[c][tid], Omax[c][tid]);
+            const uint32_t mx = max(Lmax[c][tid], ox);
             bool mark = false;
             if (v >= 64 && o >= 0 && mx != 0) {
                 mark = true;
@@ -2039,7 +2059,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         }
         lds_barrier();
         // P4: snapshot of step i+1, commit of the symbol state, marks; then the
-        // fields of step i+1 into LDS and one batch of loads for the next step
+        // fields of step i+1 into LDS and one batch of loads for the next step.
+        // The fields of step i+1, loaded a step ago, are in: one wait for every
+        // vector load (vmcnt 0) here leaves none outstanding across the
+        // branches below, so the gathers issue back to back.
+        __builtin_amdgcn_s_waitcnt(0x0F70);
         const uint64_t marked = ((uint64_t)mk[c][1] << 32) | mk[c][0];
         // entries whose region-0 state the commit of step i-1 rewrote; every
         // other entry a group owned at the start of step i-1 it still owns
@@ -2095,19 +2119,23 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             if ((int32_t)(i + 1) < (int32_t)f.size) {
                 ra0_gather(d, f.nw.y, f.nw.z, g1[q], g2[q]);
                 f.fp = ra0_fpbit(f.nw.w);
-                f.nw = ra0_word(d, f, i + 2);
             }
         }
-        if (tid < 64) {
-            qmin = 0xFFFFFFFFu; qmax = 0;
-            if (i + 2 < maxsize) {
-                if (pdone && (i + 2) % WALK_PUB == 0) ra0_wait(d, pdone, pneed, (i + 2) / WALK_PUB, wok);
-                ra0_row(gmin, gmax, stride, copies, i + 2, tid, qmin, qmax);
-            }
+        if (i + 2 < maxsize)
+            ra0_stage(d, gmin, gmax, stride, copies, i + 2, pdone, pneed, false, wok, smn, smx, wid, nwv, lane);
+        // (packed for every slot, finished ones too: no gather stays pending
+        // on any path into the next step)
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            fn[q].pk = ra0_pack(g1[q], g2[q]);
+            asm volatile("" : "+v"(fn[q].pk) : : "memory");  // packed here, before the loads below
         }
+        // the fields of step i+2 are first read at the end of step i+1: issued
+        // after every load this step waits for (loads complete in order), so
+        // that their latency overlaps step i+1
 #pragma unroll
         for (int q = 0; q < PER; ++q)
-            if ((int32_t)(i + 1) < (int32_t)fn[q].size) fn[q].pk = ra0_pack(g1[q], g2[q]);
+            if ((int32_t)(i + 1) < (int32_t)fn[q].size) fn[q].nw = ra0_word(d, fn[q], i + 2);
     }
 }
 
