@@ -1760,13 +1760,12 @@ struct Ra0Fn {
     uint64_t own;                         // snapshot registers this function may own
     uint32_t ord, rs1, rs2;               // step i (reloaded from LDS in every phase)
     uint4 nw;                             // step i+1: rd, rs1, rs2, instruction (zeros past the end)
-    uint32_t fp;                          // OP-FP bit of step i
-    uint32_t pk;                          // step i: raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18
+    uint32_t pk;                          // step i: raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18 | OP-FP << 31
 };
 
 DEV RaStep ra0_step(const Ra0Fn &f) {
-    const uint32_t fl = f.pk >> 18;
-    return ra_eval_raw(f.fp ? 0x53u : 0u, f.ord, f.rs1, f.rs2, f.pk & 0xFFu, (f.pk >> 8) & 0xFFu,
+    const uint32_t fl = (f.pk >> 18) & 3u;
+    return ra_eval_raw((f.pk >> 31) ? 0x53u : 0u, f.ord, f.rs1, f.rs2, f.pk & 0xFFu, (f.pk >> 8) & 0xFFu,
                        ((f.pk >> 16) & 1u) || (fl & 1u), ((f.pk >> 17) & 1u) || (fl & 2u), f.life);
 }
 
@@ -1783,8 +1782,9 @@ DEV void ra0_gather(const Dev &d, uint32_t rs1, uint32_t rs2, uint32_t &g1, uint
     g2 = virt_ok(d, rs2) ? d.sym[rs2 - 64u] : 0u;
 }
 
+// (bit 31 of g1, the OP-FP bit where the caller sets it, is kept)
 DEV uint32_t ra0_pack(uint32_t g1, uint32_t g2) {
-    return (g1 & 0xFFu) | ((g2 & 0xFFu) << 8) | (((g1 >> 8) & 1u) << 16) | (((g2 >> 8) & 1u) << 17);
+    return (g1 & 0x800000FFu) | ((g2 & 0xFFu) << 8) | (((g1 >> 8) & 1u) << 16) | (((g2 >> 8) & 1u) << 17);
 }
 
 // The commit of ra_commit_syms, storing only entries whose value changes:
@@ -1901,6 +1901,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
     __shared__ int changed;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const uint32_t wid = tid >> 6, nwv = nth >> 6, lane = tid & 63u;
+    // with one table copy per loading wave, a loading wave holds its copy's
+    // row of step i+2 in registers, loaded a step before it is staged
+    const uint32_t k0 = nwv - 1u - wid;
+    const bool reg_rows = nwv >= copies;
+    uint32_t rmn = 0xFFFFFFFFu, rmx = 0;
     // this workgroup's program: groups of 8 functions from fbase, L of them
     const Ra0Prog pg = progs[blockIdx.x];
     const uint32_t L = pg.L, maxsize = pg.rows, fbase = pg.fbase;  // steps: the table's rows
@@ -1922,11 +1927,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             else { atomicOr(d.flags, EF_OOB); f.j = 0; }
         }
         const uint4 w0 = ra0_word(d, f, 0);
-        f.ord = w0.x; f.rs1 = w0.y; f.rs2 = w0.z; f.fp = ra0_fpbit(w0.w);
+        f.ord = w0.x; f.rs1 = w0.y; f.rs2 = w0.z;
         f.nw = ra0_word(d, f, 1);
         uint32_t g1 = 0, g2 = 0;
         if ((int32_t)f.size > 0) ra0_gather(d, f.rs1, f.rs2, g1, g2);
-        f.pk = ra0_pack(g1, g2);
+        f.pk = ra0_pack(g1, g2) | (ra0_fpbit(w0.w) << 31);
         if (t < L) {
             const bool act = (int32_t)f.size > 0;
             cur_rd[f.j] = act ? f.ord : 0u; cur_rs1[f.j] = act ? f.rs1 : 0u; cur_rs2[f.j] = act ? f.rs2 : 0u;
@@ -1945,6 +1950,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         Omin[0][tid] = Omin[1][tid] = 0xFFFFFFFFu; Omax[0][tid] = Omax[1][tid] = 0;
     }
     if (maxsize > 1) ra0_stage(d, gmin, gmax, stride, copies, 1, pdone, pneed, true, wok, smn, smx, wid, nwv, lane);
+    if (reg_rows && k0 < copies && maxsize > 2) {
+        if (pdone && 2u % WALK_PUB == 0) ra0_wait(d, pdone, pneed, 2u / WALK_PUB, wok);
+        const uint64_t o = (uint64_t)k0 * stride + 2u * 64u + lane;
+        rmn = gmin[o]; rmx = gmax[o];
+    }
     __syncthreads();
     uint32_t fb = 0;   // wave 0: flip bits of snapshot register tid
     uint32_t om = 0xFFFFFFFFu, ox = 0;  // wave 0: owners' lookups of register tid at step i
@@ -2005,7 +2015,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 for (int q = 0; q < PER; ++q) {
                     const uint32_t j = fn[q].j;
                     if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
-                        fn[q].pk = (fn[q].pk & 0x3FFFFu) |
+                        fn[q].pk = (fn[q].pk & 0x8003FFFFu) |
                                    (ra0_flips(ra0_owned(fn[q].own, Sv[c], So[c], j), flip) << 18);
                         ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
                         ra0_lookups(Omin[c], Omax[c], ra0_step(fn[q]), j);
@@ -2118,11 +2128,14 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             g1[q] = g2[q] = 0;
             if ((int32_t)(i + 1) < (int32_t)f.size) {
                 ra0_gather(d, f.nw.y, f.nw.z, g1[q], g2[q]);
-                f.fp = ra0_fpbit(f.nw.w);
+                g1[q] |= ra0_fpbit(f.nw.w) << 31;
             }
         }
-        if (i + 2 < maxsize)
+        if (reg_rows) {
+            if (k0 < copies && i + 2 < maxsize) { smn[k0][lane] = rmn; smx[k0][lane] = rmx; }
+        } else if (i + 2 < maxsize) {
             ra0_stage(d, gmin, gmax, stride, copies, i + 2, pdone, pneed, false, wok, smn, smx, wid, nwv, lane);
+        }
         // (packed for every slot, finished ones too: no gather stays pending
         // on any path into the next step)
 #pragma unroll
@@ -2130,9 +2143,15 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             fn[q].pk = ra0_pack(g1[q], g2[q]);
             asm volatile("" : "+v"(fn[q].pk) : : "memory");  // packed here, before the loads below
         }
-        // the fields of step i+2 are first read at the end of step i+1: issued
-        // after every load this step waits for (loads complete in order), so
-        // that their latency overlaps step i+1
+        // the walk's row of step i+3, staged at the end of step i+1, and the
+        // fields of step i+2, first read at the end of step i+1: issued after
+        // every load this step waits for (loads complete in order), so that
+        // their latency overlaps step i+1
+        if (reg_rows && k0 < copies && i + 3 < maxsize) {
+            if (pdone && (i + 3) % WALK_PUB == 0) ra0_wait(d, pdone, pneed, (i + 3) / WALK_PUB, wok);
+            const uint64_t o = (uint64_t)k0 * stride + (uint64_t)(i + 3) * 64u + lane;
+            rmn = gmin[o]; rmx = gmax[o];
+        }
 #pragma unroll
         for (int q = 0; q < PER; ++q)
             if ((int32_t)(i + 1) < (int32_t)fn[q].size) fn[q].nw = ra0_word(d, fn[q], i + 2);
