@@ -716,10 +716,15 @@ DEV uint32_t stop_bits(const Seg &g, int32_t p, const int32_t *skey, uint32_t i)
 }
 
 // Per-tile counts of left and right stops over the active list.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_reduce(const Seg *segs, uint32_t m, uint32_t A,
+// The level kernels read the level's counts from the device (lv[0] active
+// elements, lv[1] ranges) and are launched on grids sized for an upper bound
+// of them; workgroups past the counts exit.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_reduce(const Seg *segs, const uint32_t *lv,
                                                                const int32_t *pivot, const int32_t *skey,
                                                                uint2 *tile_sums) {
     __shared__ uint32_t sL[SCAN_BLOCK / 64], sR[SCAN_BLOCK / 64];
+    const uint32_t A = lv[0], m = lv[1];
+    if (blockIdx.x * (uint32_t)SCAN_TILE >= A) return;
     const SegWindow wd = seg_window(segs, m, A, blockIdx.x * (uint32_t)SCAN_TILE, (uint32_t)SCAN_TILE);
     const uint32_t a0 = blockIdx.x * (uint32_t)SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t cl = 0, cr = 0;
@@ -748,9 +753,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_reduce(const Seg *segs, 
 }
 
 // One workgroup: exclusive scan of (left, right) tile sums in place; totals to *total.
-__global__ __launch_bounds__(1024) void k_scan_tiles2(uint2 *sums, uint32_t ntiles, uint2 *total) {
+__global__ __launch_bounds__(1024) void k_scan_tiles2(uint2 *sums, const uint32_t *lv, uint2 *total) {
     __shared__ uint32_t wl[16], wr[16];
     __shared__ uint32_t cl, cr;
+    const uint32_t ntiles = (uint32_t)((lv[0] + SCAN_TILE - 1) / SCAN_TILE);
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     if (threadIdx.x == 0) { cl = 0; cr = 0; }
     __syncthreads();
@@ -775,11 +781,13 @@ __global__ __launch_bounds__(1024) void k_scan_tiles2(uint2 *sums, uint32_t ntil
 
 // Compaction of the stops: Lpos[k] / Rpos[k] = position of the k-th left / right
 // stop of the level; base[s] = counts before range s.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_down(const Seg *segs, uint32_t m, uint32_t A,
+__global__ __launch_bounds__(SCAN_BLOCK) void k_sortlvl_down(const Seg *segs, const uint32_t *lv,
                                                              const int32_t *pivot, const int32_t *skey,
                                                              const uint2 *tile_pre, uint32_t *Lpos, uint32_t *Rpos,
                                                              uint2 *base) {
     __shared__ uint32_t sL[SCAN_BLOCK / 64], sR[SCAN_BLOCK / 64];
+    const uint32_t A = lv[0], m = lv[1];
+    if (blockIdx.x * (uint32_t)SCAN_TILE >= A) return;
     // the tile's stops, staged: its left and right stops are each one
     // contiguous range of Lpos / Rpos, written out coalesced at the end
     __shared__ uint32_t stL[SCAN_TILE], stR[SCAN_TILE];
@@ -861,8 +869,9 @@ DEV uint32_t uniform_final(uint32_t a, uint32_t b, uint32_t pos) {
     return pos;
 }
 
-__global__ void k_sort_search(const Seg *segs, uint32_t m, const uint2 *base, const uint2 *total,
+__global__ void k_sort_search(const Seg *segs, const uint32_t *lv, const uint2 *base, const uint2 *total,
                               const uint32_t *Lpos, const uint32_t *Rpos, SegCut *cuts) {
+    const uint32_t m = lv[1];
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m) return;
     uint32_t first = segs[s].first, last = segs[s].last;
@@ -892,9 +901,11 @@ __global__ void k_sort_search(const Seg *segs, uint32_t m, const uint2 *base, co
 // k-th-from-the-end right stop; the index within the range is the pair number.
 // A workgroup covers 256*SCAN_ITEMS active indices, consecutive lanes taking
 // consecutive pairs (coalesced stop lists).
-__global__ __launch_bounds__(256) void k_sort_swap(const Seg *segs, uint32_t m, uint32_t A, const SegCut *cuts,
+__global__ __launch_bounds__(256) void k_sort_swap(const Seg *segs, const uint32_t *lv, const SegCut *cuts,
                                                    const uint32_t *Lpos, const uint32_t *Rpos, uint32_t *sidx,
                                                    int32_t *skey) {
+    const uint32_t A = lv[0], m = lv[1];
+    if (blockIdx.x * 256u * SCAN_ITEMS >= A) return;
     const SegWindow wd = seg_window(segs, m, A, blockIdx.x * 256u * SCAN_ITEMS, 256u * SCAN_ITEMS);
     const uint32_t a0 = blockIdx.x * 256u * SCAN_ITEMS + threadIdx.x;
     if (a0 >= A) return;
@@ -911,10 +922,11 @@ __global__ __launch_bounds__(256) void k_sort_swap(const Seg *segs, uint32_t m, 
     }
 }
 
-// Level counters.  qa = next-level ranges << 32 | their active elements (reset
-// per level); ue = all-equal ranges << 32 | their elements (accumulated); one
-// 64-bit atomic per pair keeps a list's slots and offsets in the same order.
-struct SortCnt { unsigned long long qa, ue; uint32_t nheap, pad[3]; };
+// Level counters.  qa[l & 1] = ranges << 32 | active elements of level l
+// (each level's plan resets and fills the next one's); ue = all-equal ranges
+// << 32 | their elements (accumulated); one 64-bit atomic per pair keeps a
+// list's slots and offsets in the same order.
+struct SortCnt { unsigned long long qa[2], ue; uint32_t nheap, pad[3]; };
 
 DEV uint32_t block_excl_scan(uint32_t v, uint32_t *tmp /* 16 */, uint32_t &total) {
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
@@ -935,11 +947,14 @@ DEV uint32_t block_excl_scan(uint32_t v, uint32_t *tmp /* 16 */, uint32_t &total
 // Workgroups take their slots of each list with one atomic, so the lists are
 // in workgroup-arrival order; every consumer needs only that a list's slots and
 // offsets agree.
-__global__ __launch_bounds__(1024) void k_sort_plan(const Seg *segs, const SegCut *cuts, uint32_t m, Seg *next,
-                                                    int32_t *next_pivot, Seg *heap, Seg *uni, SortCnt *cnt,
-                                                    uint8_t *bound, uint32_t *sidx, int32_t *skey) {
+__global__ __launch_bounds__(1024) void k_sort_plan(const Seg *segs, const SegCut *cuts, const uint32_t *lv,
+                                                    Seg *next, int32_t *next_pivot, Seg *heap, Seg *uni,
+                                                    unsigned long long *qa_next, SortCnt *cnt, uint8_t *bound,
+                                                    uint32_t *sidx, int32_t *skey) {
     __shared__ uint32_t tmp[16];
     __shared__ uint32_t base[5];
+    const uint32_t m = lv[1];
+    if (blockIdx.x * blockDim.x >= m) return;
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     Seg g{0, 0, 0, 0};
     SegCut ct{};
@@ -956,7 +971,7 @@ __global__ __launch_bounds__(1024) void k_sort_plan(const Seg *segs, const SegCu
     const uint32_t xu = block_excl_scan(un ? 1u : 0u, tmp, tu);
     const uint32_t xe = block_excl_scan(un ? g.last - g.first : 0u, tmp, te);
     if (threadIdx.x == 0) {
-        const unsigned long long qa = (tq | ta) ? atomicAdd(&cnt->qa, ((unsigned long long)tq << 32) | ta) : 0ull;
+        const unsigned long long qa = (tq | ta) ? atomicAdd(qa_next, ((unsigned long long)tq << 32) | ta) : 0ull;
         const unsigned long long ue = (tu | te) ? atomicAdd(&cnt->ue, ((unsigned long long)tu << 32) | te) : 0ull;
         base[0] = (uint32_t)(qa >> 32); base[1] = (uint32_t)qa;
         base[2] = th ? atomicAdd(&cnt->nheap, th) : 0u;
