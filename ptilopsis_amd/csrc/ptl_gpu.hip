@@ -926,7 +926,14 @@ __global__ __launch_bounds__(256) void k_sort_swap(const Seg *segs, const uint32
 // (each level's plan resets and fills the next one's); ue = all-equal ranges
 // << 32 | their elements (accumulated); one 64-bit atomic per pair keeps a
 // list's slots and offsets in the same order.
-struct SortCnt { unsigned long long qa[2], ue; uint32_t nheap, pad[3]; };
+// nloc: ranges listed for k_sort_local; bad: a listed range of a size it does
+// not take (never, by construction; the host reports it as an internal error).
+struct SortCnt { unsigned long long qa[2], ue; uint32_t nheap, nloc, bad, pad; };
+
+// Ranges of at most SORT_LOCAL elements finish in one workgroup's LDS
+// (k_sort_local): k_sort_plan lists them (median already moved to first)
+// instead of the next level.
+constexpr uint32_t SORT_LOCAL = 2048;
 
 DEV uint32_t block_excl_scan(uint32_t v, uint32_t *tmp /* 16 */, uint32_t &total) {
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
@@ -949,10 +956,11 @@ DEV uint32_t block_excl_scan(uint32_t v, uint32_t *tmp /* 16 */, uint32_t &total
 // offsets agree.
 __global__ __launch_bounds__(1024) void k_sort_plan(const Seg *segs, const SegCut *cuts, const uint32_t *lv,
                                                     Seg *next, int32_t *next_pivot, Seg *heap, Seg *uni,
+                                                    Seg *loc, int32_t *loc_pivot,
                                                     unsigned long long *qa_next, SortCnt *cnt, uint8_t *bound,
                                                     uint32_t *sidx, int32_t *skey) {
     __shared__ uint32_t tmp[16];
-    __shared__ uint32_t base[5];
+    __shared__ uint32_t base[6];
     const uint32_t m = lv[1];
     if (blockIdx.x * blockDim.x >= m) return;
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -962,9 +970,12 @@ __global__ __launch_bounds__(1024) void k_sort_plan(const Seg *segs, const SegCu
     const bool v = s < m, un = v && ct.uni;
     const uint32_t lim = g.limit - 1;
     const uint32_t c0 = ct.cut - g.first, c1 = g.last - ct.cut;
-    const bool n0 = v && !un && c0 > 16 && lim > 0, n1 = v && !un && c1 > 16 && lim > 0;
+    const bool r0 = v && !un && c0 > 16 && lim > 0, r1 = v && !un && c1 > 16 && lim > 0;
+    const bool n0 = r0 && c0 > SORT_LOCAL, n1 = r1 && c1 > SORT_LOCAL;    // next level
+    const bool l0 = r0 && c0 <= SORT_LOCAL, l1 = r1 && c1 <= SORT_LOCAL;  // finished in LDS
     const bool h0b = v && !un && c0 > 16 && lim == 0, h1b = v && !un && c1 > 16 && lim == 0;
-    uint32_t tq, ta, th, tu, te;
+    uint32_t tq, ta, th, tu, te, tl;
+    const uint32_t xl = block_excl_scan((l0 ? 1u : 0u) + (l1 ? 1u : 0u), tmp, tl);
     const uint32_t xq = block_excl_scan((n0 ? 1u : 0u) + (n1 ? 1u : 0u), tmp, tq);
     const uint32_t xa = block_excl_scan((n0 ? c0 : 0u) + (n1 ? c1 : 0u), tmp, ta);
     const uint32_t xh = block_excl_scan((h0b ? 1u : 0u) + (h1b ? 1u : 0u), tmp, th);
@@ -976,6 +987,7 @@ __global__ __launch_bounds__(1024) void k_sort_plan(const Seg *segs, const SegCu
         base[0] = (uint32_t)(qa >> 32); base[1] = (uint32_t)qa;
         base[2] = th ? atomicAdd(&cnt->nheap, th) : 0u;
         base[3] = (uint32_t)(ue >> 32); base[4] = (uint32_t)ue;
+        base[5] = tl ? atomicAdd(&cnt->nloc, tl) : 0u;
     }
     __syncthreads();
     if (!v) return;
@@ -988,8 +1000,151 @@ __global__ __launch_bounds__(1024) void k_sort_plan(const Seg *segs, const SegCu
     uint32_t q = base[0] + xq, a = base[1] + xa, h = base[2] + xh;
     if (n0) { next[q] = Seg{g.first, ct.cut, lim, a}; next_pivot[q] = median_to_first(g.first, ct.cut, sidx, skey); ++q; a += c0; }
     if (n1) { next[q] = Seg{ct.cut, g.last, lim, a}; next_pivot[q] = median_to_first(ct.cut, g.last, sidx, skey); }
+    uint32_t lq = base[5] + xl;
+    if (l0) { loc[lq] = Seg{g.first, ct.cut, lim, 0}; loc_pivot[lq] = median_to_first(g.first, ct.cut, sidx, skey); ++lq; }
+    if (l1) { loc[lq] = Seg{ct.cut, g.last, lim, 0}; loc_pivot[lq] = median_to_first(ct.cut, g.last, sidx, skey); }
     if (h0b) heap[h++] = Seg{g.first, ct.cut, 0, 0};
     if (h1b) heap[h++] = Seg{ct.cut, g.last, 0, 0};
+}
+
+// One listed range per workgroup, all its remaining levels in LDS: the same
+// partitions as the level kernels above (stops, pairs, cut, all-equal ranges
+// in closed form when their recursion fits the limit), the same outputs
+// (boundaries in `bound`, entries of the heap and all-equal lists), its
+// subranges kept in position order.  Thread t holds positions [8t, 8t + 8).
+constexpr uint32_t SORT_LOCAL_SUBS = SORT_LOCAL / 17 + 2;
+__global__ __launch_bounds__(256) void k_sort_local(const Seg *loc, const int32_t *loc_pivot, const SortCnt *cntr,
+                                                    uint32_t *sidx, int32_t *skey, uint8_t *bound, Seg *heap,
+                                                    Seg *uni, SortCnt *cnt) {
+    static_assert(SORT_LOCAL == 256 * 8, "8 positions per thread");
+    __shared__ int32_t K[SORT_LOCAL];
+    __shared__ uint32_t X[SORT_LOCAL];
+    __shared__ uint16_t LP[SORT_LOCAL], RP[SORT_LOCAL];
+    __shared__ uint16_t sf[2][SORT_LOCAL_SUBS], sl[2][SORT_LOCAL_SUBS];
+    __shared__ uint8_t slim[2][SORT_LOCAL_SUBS];
+    __shared__ int32_t spiv[2][SORT_LOCAL_SUBS];
+    __shared__ uint16_t bL[SORT_LOCAL_SUBS], bR[SORT_LOCAL_SUBS], cLs[SORT_LOCAL_SUBS], cRs[SORT_LOCAL_SUBS];
+    __shared__ uint16_t mms[SORT_LOCAL_SUBS], cts[SORT_LOCAL_SUBS];
+    __shared__ uint8_t unis[SORT_LOCAL_SUBS];
+    __shared__ uint32_t tmp[16];
+    if (blockIdx.x >= cntr->nloc) return;
+    const uint32_t tid = threadIdx.x;
+    const Seg g = loc[blockIdx.x];
+    const uint32_t base = g.first, len = g.last - g.first;
+    if (len > SORT_LOCAL || len < 17) { if (tid == 0) atomicOr(&cnt->bad, 1u); return; }
+    for (uint32_t p = tid; p < len; p += 256) { K[p] = skey[base + p]; X[p] = sidx[base + p]; }
+    if (tid == 0) { sf[0][0] = 0; sl[0][0] = (uint16_t)len; slim[0][0] = (uint8_t)g.limit; spiv[0][0] = loc_pivot[blockIdx.x]; }
+    uint32_t cur = 0, nsub = 1;
+    __syncthreads();
+    const uint32_t p0 = tid * 8u;
+    while (nsub > 0) {
+        // stops of this thread's positions: subrange walk from the last one starting at or before p0
+        uint32_t s0 = 0;
+        {
+            uint32_t lo = 0, hi = nsub;
+            while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (sf[cur][mid] <= p0) lo = mid; else hi = mid; }
+            s0 = lo;
+        }
+        uint32_t bits = 0, cl = 0, cr = 0;
+        {
+            uint32_t s = s0;
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t p = p0 + k;
+                if (p >= len) break;
+                while (s + 1 < nsub && sf[cur][s + 1] <= p) ++s;
+                if (p < sf[cur][s] || p >= sl[cur][s] || p == sf[cur][s]) continue;
+                const int32_t kk = K[p], pv = spiv[cur][s];
+                const uint32_t b = (!(kk < pv) ? 1u : 0u) | (!(pv < kk) ? 2u : 0u);
+                bits |= b << (2 * k);
+                cl += b & 1u; cr += b >> 1;
+            }
+        }
+        uint32_t totL, totR;
+        uint32_t rl = block_excl_scan(cl, tmp, totL);
+        uint32_t rr = block_excl_scan(cr, tmp, totR);
+        const uint32_t rl0 = rl;
+        {
+            uint32_t s = s0;
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t p = p0 + k;
+                if (p >= len) break;
+                while (s + 1 < nsub && sf[cur][s + 1] <= p) ++s;
+                if (p == sf[cur][s]) { bL[s] = (uint16_t)rl; bR[s] = (uint16_t)rr; }
+                const uint32_t b = (bits >> (2 * k)) & 3u;
+                if (b & 1u) LP[rl++] = (uint16_t)p;
+                if (b & 2u) RP[rr++] = (uint16_t)p;
+            }
+        }
+        __syncthreads();
+        // per subrange: stop counts, pairs, cut (k_sort_search)
+        if (tid < nsub) {
+            const uint32_t s = tid, f = sf[cur][s], l = sl[cur][s];
+            const uint32_t eL = s + 1 < nsub ? bL[s + 1] : totL, eR = s + 1 < nsub ? bR[s + 1] : totR;
+            const uint32_t b0 = bL[s], b1 = bR[s], cL = eL - b0, cR = eR - b1;
+            uint32_t lo = 0, hi = cL < cR ? cL : cR;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) / 2;
+                if (LP[b0 + mid - 1] < RP[b1 + cR - mid]) lo = mid; else hi = mid - 1;
+            }
+            uint32_t cut = 0xFFFFFFFFu;
+            if (lo + 1 <= cL) cut = LP[b0 + lo];
+            if (lo >= 1) { const uint32_t r = RP[b1 + cR - lo]; if (r < cut) cut = r; }
+            if (cut <= f || cut >= l) cut = f + 1;
+            const uint32_t n = l - f;
+            const bool un = cL == n - 1 && cR == n - 1 && uniform_depth(n) <= slim[cur][s];
+            cLs[s] = (uint16_t)cL; cRs[s] = (uint16_t)cR;
+            mms[s] = (uint16_t)(un ? 0u : lo); cts[s] = (uint16_t)(un ? l : cut); unis[s] = un ? 1 : 0;
+        }
+        __syncthreads();
+        // the pairs' swaps, each by the thread holding its left stop
+        {
+            uint32_t s = s0, r = rl0;
+            for (uint32_t k = 0; k < 8; ++k) {
+                const uint32_t p = p0 + k;
+                if (p >= len) break;
+                while (s + 1 < nsub && sf[cur][s + 1] <= p) ++s;
+                if (!((bits >> (2 * k)) & 1u)) continue;
+                const uint32_t kk = r++ - bL[s];
+                if (kk < mms[s]) {
+                    const uint32_t q = RP[bR[s] + cRs[s] - 1u - kk];
+                    const int32_t tk = K[p]; K[p] = K[q]; K[q] = tk;
+                    const uint32_t tx = X[p]; X[p] = X[q]; X[q] = tx;
+                }
+            }
+        }
+        __syncthreads();
+        // children (k_sort_plan): the next subranges in position order
+        const uint32_t nx = cur ^ 1u;
+        uint32_t nc = 0, f = 0, l = 0, cut = 0, lim = 0;
+        bool c0n = false, c1n = false;
+        if (tid < nsub) {
+            f = sf[cur][tid]; l = sl[cur][tid]; cut = cts[tid];
+            const uint32_t limit = slim[cur][tid];
+            lim = limit - 1u;
+            if (unis[tid]) {
+                const unsigned long long ue = atomicAdd(&cnt->ue, (1ull << 32) | (unsigned long long)(l - f));
+                uni[(uint32_t)(ue >> 32)] = Seg{base + f, base + l, limit, (uint32_t)ue};
+                bound[base + f] = 2;
+            } else {
+                bound[base + cut] = 1;
+                const uint32_t c0 = cut - f, c1 = l - cut;
+                c0n = c0 > 16 && lim > 0; c1n = c1 > 16 && lim > 0;
+                if (c0 > 16 && lim == 0) heap[atomicAdd(&cnt->nheap, 1u)] = Seg{base + f, base + cut, 0, 0};
+                if (c1 > 16 && lim == 0) heap[atomicAdd(&cnt->nheap, 1u)] = Seg{base + cut, base + l, 0, 0};
+                nc = (c0n ? 1u : 0u) + (c1n ? 1u : 0u);
+            }
+        }
+        uint32_t tot;
+        uint32_t q = block_excl_scan(nc, tmp, tot);
+        if (c0n) { sf[nx][q] = (uint16_t)f; sl[nx][q] = (uint16_t)cut; slim[nx][q] = (uint8_t)lim; ++q; }
+        if (c1n) { sf[nx][q] = (uint16_t)cut; sl[nx][q] = (uint16_t)l; slim[nx][q] = (uint8_t)lim; }
+        __syncthreads();
+        if (tid < tot) spiv[nx][tid] = median_to_first(sf[nx][tid], sl[nx][tid], X, K);
+        __syncthreads();
+        cur = nx;
+        nsub = tot;
+    }
+    for (uint32_t p = tid; p < len; p += 256) { skey[base + p] = K[p]; sidx[base + p] = X[p]; }
 }
 
 DEV bool KLESS(const int32_t *skey, uint32_t a, uint32_t b) { return skey[a] < skey[b]; }
