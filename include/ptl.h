@@ -192,6 +192,15 @@ int ptl_profile_get(ptl_ctx *ctx, int idx, const char **name, uint64_t *total_ns
  * for the known-answer test of benchmarks/prefix_sum.cpp:35-40,215-226. */
 int ptl_debug_exclusive_scan(ptl_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n);
 
+/* The device depth sort on its own: each program's node range
+ * [bounds[p], bounds[p+1]) of `keys` is ordered as std::sort orders node ids
+ * by depth in rv_generator_avx.cpp:539-541 (libstdc++ introsort), and
+ * out[position] = node id.  bounds[0] = 0, bounds[n_programs] = n; every
+ * program non-empty (PTL_ERR_ARG otherwise).  For tests of the sort on chosen
+ * key patterns; a compile runs it itself. */
+int ptl_debug_depth_sort(ptl_ctx *ctx, const int32_t *keys, const uint32_t *bounds, uint32_t n_programs,
+                         uint32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,7 @@ EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_uplo
            "ptl_server_release", "ptl_server_stats",
            "ptl_result_asm", "ptl_disassemble", "ptl_tune_set", "ptl_tune_get",
            "ptl_debug_dump", "ptl_debug_stop_after", "ptl_debug_overlap",
-           "ptl_debug_exclusive_scan", "ptl_profile_enable", "ptl_profile_get")
+           "ptl_debug_exclusive_scan", "ptl_debug_depth_sort", "ptl_profile_enable", "ptl_profile_get")
 
 
 class PtlAst(C.Structure):
@@ -93,6 +93,8 @@ def bind(lib):
     lib.ptl_debug_overlap.restype = C.c_int
     lib.ptl_debug_exclusive_scan.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
     lib.ptl_debug_exclusive_scan.restype = C.c_int
+    lib.ptl_debug_depth_sort.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+    lib.ptl_debug_depth_sort.restype = C.c_int
     lib.ptl_profile_enable.argtypes = [C.c_void_p, C.c_int]
     lib.ptl_profile_enable.restype = C.c_int
     lib.ptl_profile_get.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),

@@ -175,6 +175,17 @@ class Context:
             raise PtlError(s)
         return out
 
+    def depth_sort(self, keys: np.ndarray, bounds) -> np.ndarray:
+        """The device depth sort of each program range [bounds[p], bounds[p+1]) of
+        `keys`: node ids by position (ptl_debug_depth_sort)."""
+        k = np.ascontiguousarray(keys, dtype=np.int32)
+        b = np.ascontiguousarray(bounds, dtype=np.uint32)
+        out = np.zeros(k.shape[0], np.uint32)
+        s = self.lib.ptl_debug_depth_sort(self.h, k.ctypes.data, b.ctypes.data, b.shape[0] - 1, out.ctypes.data)
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return out
+
     def dump(self, name: str) -> np.ndarray:
         n = self.lib.ptl_debug_dump(self.h, name.encode(), None, 0)
         if n < 0:
