@@ -371,7 +371,7 @@ def main() -> int:
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: the code generator has no multi-GPU path "
-              "(one program per device; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(it would shard by program, whole programs per device, with no collectives)", file=sys.stderr)
         return 2
 
     from ptilopsis_amd.generator import Context
