@@ -2143,6 +2143,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 const RaStep r = ra0_step(f);
                 if (f.own) {
                     ra0_lookups(Omin[c], Omax[c], r, j);
+                    rrs[j] = r.r1r | (r.r2r << 8) | (r.rdr << 16);  // final unless a settle round re-evaluates
                 } else {
                     ra0_lookups(Lmin[c], Lmax[c], r, j);
                     rrs[j] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
@@ -2188,7 +2189,9 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                         fn[q].pk = (fn[q].pk & 0x8003FFFFu) |
                                    (ra0_flips(ra0_owned(fn[q].own, Sv[c], So[c], j), flip) << 18);
                         ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
-                        ra0_lookups(Omin[c], Omax[c], ra0_step(fn[q]), j);
+                        const RaStep r = ra0_step(fn[q]);
+                        ra0_lookups(Omin[c], Omax[c], r, j);
+                        rrs[j] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
                     }
                 }
                 lds_barrier();
@@ -2211,16 +2214,20 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 if (round > L + 1) { if (tid == 0) atomicOr(d.flags, EF_OOB); break; }
             }
         }
-        // P3: possible owners finalize; marks on the snapshot's virtual
-        // registers; the snapshot of step i+1 starts as a copy
+        // P3: possible owners finalize, from the registers of their last
+        // evaluation (P1's, or the last settle round's: the same inputs, so the
+        // same result) and the lifetime update of ra_eval_raw; marks on the
+        // snapshot's virtual registers; the snapshot of step i+1 starts as a copy
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const uint32_t j = fn[q].j;
             if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
-                ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
-                const RaStep r = ra0_step(fn[q]);
-                rrs[j] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
-                fn[q].life = r.newlife;
+                const uint32_t rr = rrs[j];
+                RaStep r;
+                r.ord = cur_rd[j];
+                r.r1r = rr & 0xFFu; r.r2r = (rr >> 8) & 0xFFu; r.rdr = (rr >> 16) & 0xFFu;
+                const uint64_t clr = ((1ull << r.r1r) | (1ull << r.r2r)) & ~1ull;
+                fn[q].life = (fn[q].life & ~clr) | (1ull << r.rdr);
                 ra0_keys(wkey[c], r, j);
             }
         }
