@@ -1249,32 +1249,48 @@ __global__ __launch_bounds__(256) void k_opt_init(Dev d) {
 // so the second case does not arise there.)
 constexpr uint32_t OPT_RANGE = 256 * 8;
 
+// The range's fields and its used/store flags are staged in LDS once; the
+// rounds read them there (flags of other ranges from HBM).  A clear is written
+// to both (flags only go from 1 to 0, so HBM never takes back a clear made by
+// another workgroup); a clear another workgroup makes in this range during the
+// launch is a cross clear, which asks for the next launch, where it is staged.
 __global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
     __shared__ int local;
+    __shared__ uint32_t s_rd[OPT_RANGE], s_r1[OPT_RANGE], s_r2[OPT_RANGE];
+    __shared__ uint8_t s_used[OPT_RANGE], s_st[OPT_RANGE];
     const uint64_t b0 = (uint64_t)blockIdx.x * OPT_RANGE;
     const uint64_t b1 = b0 + OPT_RANGE < d.W0 ? b0 + OPT_RANGE : d.W0;
+    const uint32_t nr = (uint32_t)(b1 - b0);
+    for (uint32_t k = threadIdx.x; k < nr; k += 256) {
+        s_rd[k] = d.w0_rd[b0 + k]; s_r1[k] = d.w0_rs1[b0 + k]; s_r2[k] = d.w0_rs2[b0 + k];
+        s_used[k] = d.used[b0 + k]; s_st[k] = d.sstore[b0 + k];
+    }
+    auto used_of = [&](uint32_t x) -> bool { return (uint64_t)x - b0 < nr ? s_used[x - b0] != 0 : d.used[x] != 0; };
+    auto store_of = [&](uint32_t x) -> bool { return (uint64_t)x - b0 < nr ? s_st[x - b0] != 0 : d.sstore[x] != 0; };
     bool cross = false;  // a clear outside the range, or a destination outside it
     bool settled = false;
     for (int round = 0; round < 64; ++round) {
         if (threadIdx.x == 0) local = 0;
         __syncthreads();
         bool loc = false;
-        for (uint64_t i = b1 - 1 - threadIdx.x; (int64_t)i >= (int64_t)b0; i -= 256) {
-            const uint32_t rd = d.w0_rd[i];
+        for (int32_t k = (int32_t)nr - 1 - (int32_t)threadIdx.x; k >= 0; k -= 256) {
+            const uint32_t rd = s_rd[k];
             if (!((int32_t)rd > 63)) continue;
             const uint32_t v = rd - 64u;
             if (v >= d.W0) continue;
-            cross |= (uint64_t)v - b0 >= b1 - b0;
-            if (d.used[v]) continue;
-            const int32_t a = (int32_t)(d.w0_rs1[i] - 64u), b = (int32_t)(d.w0_rs2[i] - 64u);
-            if (a >= 0 && (uint32_t)a < d.W0 && !d.sstore[a] && d.used[a]) {
+            cross |= (uint64_t)v - b0 >= nr;
+            if (used_of(v)) continue;
+            const int32_t a = (int32_t)(s_r1[k] - 64u), b = (int32_t)(s_r2[k] - 64u);
+            if (a >= 0 && (uint32_t)a < d.W0 && !store_of((uint32_t)a) && used_of((uint32_t)a)) {
+                const bool in = (uint64_t)(uint32_t)a - b0 < nr;
+                if (in) s_used[(uint32_t)a - b0] = 0;
                 d.used[a] = 0;
-                const bool in = (uint64_t)a >= b0 && (uint64_t)a < b1;
                 loc |= in; cross |= !in;
             }
-            if (b >= 0 && (uint32_t)b < d.W0 && !d.sstore[b] && d.used[b]) {
+            if (b >= 0 && (uint32_t)b < d.W0 && !store_of((uint32_t)b) && used_of((uint32_t)b)) {
+                const bool in = (uint64_t)(uint32_t)b - b0 < nr;
+                if (in) s_used[(uint32_t)b - b0] = 0;
                 d.used[b] = 0;
-                const bool in = (uint64_t)b >= b0 && (uint64_t)b < b1;
                 loc |= in; cross |= !in;
             }
         }
