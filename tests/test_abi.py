@@ -37,3 +37,18 @@ def test_strerror_without_a_device():
     lib.ptl_strerror.restype = C.c_char_p
     assert lib.ptl_strerror(0) == b"ok"
     assert b"DepthTree" in lib.ptl_strerror(-2)
+
+
+def test_entry_points_refuse_a_null_context_without_a_device():
+    # argument checks come before any HIP call, so they hold on a host with no GPU
+    if not os.path.exists(HIP_SO):
+        pytest.skip("libptl_hip.so not built")
+    from ptilopsis_amd import abi
+    lib = abi.bind(C.CDLL(HIP_SO))
+    keys = (C.c_int32 * 4)(3, 1, 2, 0)
+    bounds = (C.c_uint32 * 2)(0, 4)
+    out = (C.c_uint32 * 4)()
+    arg = next(k for k, v in abi.ERRORS.items() if v == "PTL_ERR_ARG")
+    assert lib.ptl_debug_depth_sort(None, keys, bounds, 1, out) == arg
+    assert lib.ptl_debug_exclusive_scan(None, out, out, 4) == arg
+    assert lib.ptl_debug_overlap(None, 1) == arg
