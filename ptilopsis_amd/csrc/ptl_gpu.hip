@@ -1923,11 +1923,11 @@ __global__ __launch_bounds__(RA0_THREADS) void k_ra_lane0_generic(Dev d, const R
 // step i.  The fields of step i sit in LDS (cur_*), where the snapshot checks
 // read them too.  A function's virtual registers are read and written only by
 // its own thread (marks by other groups reach the owner through LDS), so the
-// barriers of a step order LDS alone.  Each step ends with one batch of loads
-// for the next (operand state and fields of all functions of the thread) and a
-// single wait for it.  Per-step LDS tables are double-buffered by step parity
-// so that no barrier is needed between the end of a step and the start of the
-// next.
+// barriers of a step order LDS alone.  Each step ends with the loads for the
+// next: the operands' symbol states (waited for), the walk's row two steps
+// ahead (by the loading waves) and the fields two steps ahead (waited for a
+// step later).  Per-step LDS tables are double-buffered by step parity so that
+// no barrier is needed between the end of a step and the start of the next.
 //
 // Snapshot owners: only a group owning an entry of the region-0 snapshot (a
 // virtual register it wrote there) can see a swap made earlier in the same
