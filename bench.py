@@ -268,11 +268,31 @@ def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: i
     return out
 
 
+OUT_FIELDS = ("words", "func_ids", "func_starts", "func_ends", "func_sizes")
+
+
+def first_difference(got: dict, want: dict):
+    """None when every output array is equal, else the first differing field,
+    its first differing index and the two values there (-1: past the end)."""
+    for k in OUT_FIELDS:
+        g, w = np.asarray(got[k]), np.asarray(want[k])
+        if g.shape == w.shape and np.array_equal(g, w):
+            continue
+        m = min(g.shape[0], w.shape[0])
+        idx = np.nonzero(g[:m] != w[:m])[0]
+        i = int(idx[0]) if idx.size else m
+        return {"field": k, "index": i, "got": int(g[i]) if i < g.shape[0] else -1,
+                "want": int(w[i]) if i < w.shape[0] else -1, "got_len": int(g.shape[0]), "want_len": int(w.shape[0])}
+    return None
+
+
 def measure_serving(rates, seconds: float, pool_n: int) -> dict:
     """Serving stream: Poisson arrivals at each rate for `seconds`, requests
     drawn in order from a pool of programs of log-normal size, submitted to the
     library's request loop (ptl_server) from a host thread as they arrive.  Each
-    result is checked against the same program compiled alone."""
+    result is checked against the same program compiled alone; a mismatch is
+    recorded with its ticket, batch, worker, the batch's members and the first
+    differing word."""
     import threading
     from ptilopsis_amd import workloads as W
     from ptilopsis_amd.generator import Context, Server
@@ -280,8 +300,7 @@ def measure_serving(rates, seconds: float, pool_n: int) -> dict:
     pool = W.serve_pool(pool_n)
     log(f"serving pool: {pool_n} programs, {sum(t.n for t in pool)} nodes (generated+parsed in {time.time() - t0:.1f}s)")
     ctx = Context(0)
-    key = lambda r: hash(r["words"].tobytes() + r["func_starts"].tobytes() + r["func_ids"].tobytes())
-    ref = [key(ctx.compile(t)) for t in pool]
+    ref = [{k: v for k, v in ctx.compile(t).items() if k in OUT_FIELDS} for t in pool]
     ctx.close()
     srv = Server(0, hbm_budget=48 << 30, max_batch=4096, workers=2)
     for k in [srv.submit(t) for t in pool[:64]]:  # workers' arenas warm
@@ -302,26 +321,42 @@ def measure_serving(rates, seconds: float, pool_n: int) -> dict:
         th = threading.Thread(target=submitter)
         tb = time.perf_counter()
         th.start()
-        lat, qlat, ok, nodes = [], [], True, 0
+        lat, nodes, mism = [], 0, []
+        phases = {k: [] for k in ("queue_ns", "pack_ns", "run_ns", "device_ns", "copyout_ns")}
+        members = {}  # batch id -> [(ticket, pool index)]
         for i in range(len(arr)):
             while tickets[i] is None:
                 time.sleep(0.00005)
             r, tm = srv.wait(tickets[i])
             lat.append(tm["total_ns"])
-            qlat.append(tm["queue_ns"])
-            ok &= key(r) == ref[i % pool_n]
+            for k in phases:
+                phases[k].append(tm[k])
+            members.setdefault(tm["batch_id"], []).append((tickets[i], i % pool_n))
+            d = first_difference(r, ref[i % pool_n])
+            if d is not None:
+                d.update(ticket=tickets[i], request=i, pool_index=i % pool_n, nodes=pool[i % pool_n].n,
+                         batch_id=tm["batch_id"], worker=tm["worker"], batch_programs=tm["batch_programs"])
+                mism.append(d)
+                log(f"serving MISMATCH: {d}")
             nodes += pool[i % pool_n].n
         th.join()
         el = time.perf_counter() - tb
         st1 = srv.stats()
         nb = st1["batches"] - st0["batches"]
+        for d in mism:
+            d["batch_members"] = members.get(d["batch_id"], [])
+        pct = lambda v, q: float(np.percentile(v, q)) / 1e6
         rec = {"rate_offered": rate, "requests": len(arr), "seconds": el,
                "rate_achieved": len(arr) / max(1e-9, float(arr[-1]) if len(arr) else 1.0),
-               "p50_ms": float(np.percentile(lat, 50)) / 1e6, "p99_ms": float(np.percentile(lat, 99)) / 1e6,
-               "p99_queue_ms": float(np.percentile(qlat, 99)) / 1e6, "nodes_per_s": nodes / el,
-               "batches": nb, "mean_batch": len(arr) / max(1, nb), "same_as_alone": bool(ok)}
+               "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "nodes_per_s": nodes / el,
+               "batches": nb, "mean_batch": len(arr) / max(1, nb),
+               "breakdown_ms": {k[:-3]: {"p50": pct(v, 50), "p99": pct(v, 99)} for k, v in phases.items()},
+               "same_as_alone": not mism, "mismatches": mism}
+        b = rec["breakdown_ms"]
         log(f"serving {rate} req/s: p50 {rec['p50_ms']:.2f} ms, p99 {rec['p99_ms']:.2f} ms, "
-            f"{rec['nodes_per_s'] / 1e6:.1f} M nodes/s, {nb} batches (mean {rec['mean_batch']:.1f}), same as alone: {ok}")
+            f"{rec['nodes_per_s'] / 1e6:.1f} M nodes/s, {nb} batches (mean {rec['mean_batch']:.1f}), "
+            f"mismatches: {len(mism)}; p50 queue {b['queue']['p50']:.2f} pack {b['pack']['p50']:.2f} "
+            f"run {b['run']['p50']:.2f} (device {b['device']['p50']:.2f}) copyout {b['copyout']['p50']:.2f} ms")
         out.append(rec)
     srv.close()
     return {"pool_programs": pool_n, "pool_nodes": sum(t.n for t in pool), "seconds_per_rate": seconds,

@@ -121,6 +121,12 @@ typedef struct {
     uint64_t queue_ns;            /* submit to the start of its batch */
     uint64_t total_ns;            /* submit to results ready */
     uint32_t batch_programs;      /* programs in the batch it ran in */
+    uint32_t worker;              /* worker (context) that ran the batch */
+    uint64_t batch_id;            /* 1-based sequence number of the batch in this server */
+    uint64_t pack_ns;             /* host: the batch's arrays concatenated */
+    uint64_t run_ns;              /* upload, the seven stages and their host synchronisations */
+    uint64_t device_ns;           /* of run_ns: the stages' device time (sum of stage_ns) */
+    uint64_t copyout_ns;          /* results copied back and split per request */
 } ptl_serve_times;
 int ptl_server_create(ptl_server **out, int hip_device, uint64_t hbm_budget_bytes, uint32_t max_batch,
                       uint32_t workers);

@@ -42,7 +42,9 @@ class PtlResult(C.Structure):
 
 
 class PtlServeTimes(C.Structure):
-    _fields_ = [("queue_ns", C.c_uint64), ("total_ns", C.c_uint64), ("batch_programs", C.c_uint32)]
+    _fields_ = [("queue_ns", C.c_uint64), ("total_ns", C.c_uint64), ("batch_programs", C.c_uint32),
+                ("worker", C.c_uint32), ("batch_id", C.c_uint64), ("pack_ns", C.c_uint64), ("run_ns", C.c_uint64),
+                ("device_ns", C.c_uint64), ("copyout_ns", C.c_uint64)]
 
 
 def bind(lib):

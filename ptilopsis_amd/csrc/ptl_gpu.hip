@@ -1254,8 +1254,17 @@ constexpr uint32_t OPT_RANGE = 256 * 8;
 // to both (flags only go from 1 to 0, so HBM never takes back a clear made by
 // another workgroup); a clear another workgroup makes in this range during the
 // launch is a cross clear, which asks for the next launch, where it is staged.
+//
+// The round's "some word of the range lost its use" flag is double-buffered by
+// round parity.  Thread 0 clears local[(round + 1) & 1] at the top of the next
+// round while slower waves may still be reading local[round & 1] after the
+// round's last barrier; it can only come back to local[round & 1] (two rounds
+// later) after every wave has passed the next round's first barrier, i.e.
+// after every wave has read it.  With a single flag, a wave still to read it
+// could see the next round's clear, leave the loop and drop its words from
+// the later rounds, and the launch could then report a fixpoint early.
 __global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
-    __shared__ int local;
+    __shared__ int local[2];
     __shared__ uint32_t s_rd[OPT_RANGE], s_r1[OPT_RANGE], s_r2[OPT_RANGE];
     __shared__ uint8_t s_used[OPT_RANGE], s_st[OPT_RANGE];
     const uint64_t b0 = (uint64_t)blockIdx.x * OPT_RANGE;
@@ -1270,7 +1279,7 @@ __global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
     bool cross = false;  // a clear outside the range, or a destination outside it
     bool settled = false;
     for (int round = 0; round < 64; ++round) {
-        if (threadIdx.x == 0) local = 0;
+        if (threadIdx.x == 0) local[round & 1] = 0;
         __syncthreads();
         bool loc = false;
         for (int32_t k = (int32_t)nr - 1 - (int32_t)threadIdx.x; k >= 0; k -= 256) {
@@ -1294,9 +1303,9 @@ __global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
                 loc |= in; cross |= !in;
             }
         }
-        if (loc) local = 1;
+        if (loc) local[round & 1] = 1;
         __syncthreads();
-        if (!local) { settled = true; break; }
+        if (!local[round & 1]) { settled = true; break; }
     }
     if (cross || (!settled && threadIdx.x == 0)) *changed = 1;
 }

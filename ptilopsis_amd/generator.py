@@ -228,8 +228,7 @@ class Server:
             out = result_dict(r)
         finally:
             self.lib.ptl_server_release(self.h, ticket)
-        return out, {"queue_ns": int(tm.queue_ns), "total_ns": int(tm.total_ns),
-                     "batch_programs": int(tm.batch_programs)}
+        return out, {k: int(getattr(tm, k)) for k, _ in abi.PtlServeTimes._fields_}
 
     def stats(self) -> dict:
         v = [C.c_uint64() for _ in range(4)]

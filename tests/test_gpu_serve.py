@@ -20,10 +20,42 @@ def say(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def assert_same(gpu, ref, what=""):
+def first_difference(gpu, ref):
+    """None when every output array is equal, else the first differing field,
+    index and values (-1: past the end)."""
     for k in OUT:
         g, r = np.asarray(gpu[k]), np.asarray(ref[k])
-        assert g.shape == r.shape and np.array_equal(g, r), (what, k)
+        if g.shape == r.shape and np.array_equal(g, r):
+            continue
+        m = min(g.shape[0], r.shape[0])
+        idx = np.nonzero(g[:m] != r[:m])[0]
+        i = int(idx[0]) if idx.size else m
+        return {"field": k, "index": i, "got": int(g[i]) if i < g.shape[0] else -1,
+                "want": int(r[i]) if i < r.shape[0] else -1, "got_len": g.shape[0], "want_len": r.shape[0]}
+    return None
+
+
+def assert_same(gpu, ref, what=""):
+    d = first_difference(gpu, ref)
+    assert d is None, (what, d)
+
+
+def check_served(got, alone, tickets, nodes):
+    """Every served result equals its alone compile; on a mismatch the message
+    names the request, its ticket, batch, worker, the batch's members and the
+    first differing word."""
+    members = {}
+    for i, (_, tm) in enumerate(got):
+        members.setdefault(tm["batch_id"], []).append((tickets[i], i))
+    bad = []
+    for i, ((r, tm), a) in enumerate(zip(got, alone)):
+        d = first_difference(r, a)
+        if d is not None:
+            d.update(request=i, ticket=tickets[i], nodes=nodes[i], batch_id=tm["batch_id"], worker=tm["worker"],
+                     batch_programs=tm["batch_programs"], batch_members=members[tm["batch_id"]])
+            say("MISMATCH", d)
+            bad.append(d)
+    assert not bad, bad
 
 
 def mixed_requests():
@@ -48,15 +80,14 @@ def test_gpu_serve_mix_equals_alone(gpu_ctx):
     say("stats", st)
     assert st["requests_done"] == len(reqs) and st["largest_batch"] > 1
     assert max(tm["batch_programs"] for _, tm in got) > 1
-    for p, ((r, tm), a, t) in enumerate(zip(got, alone, reqs)):
-        assert_same(r, a, f"request {p}")
+    check_served(got, alone, tickets, [t.n for t in reqs])
     for p in (0, 1, 5, len(reqs) - 2):
         assert_same(got[p][0], oracle.compile_tree(reqs[p]), f"request {p} vs oracle")
 
 
 def test_gpu_serve_continuous_arrivals(gpu_ctx):
     pool = W.serve_pool(48)
-    alone = [gpu_ctx.compile(t)["words"] for t in pool]
+    alone = [gpu_ctx.compile(t) for t in pool]
     arr = W.arrivals(2000, 0.4)
     say("continuous arrivals:", len(arr), "requests over 0.4 s")
     srv = Server(0, max_batch=256, workers=2)
@@ -69,19 +100,21 @@ def test_gpu_serve_continuous_arrivals(gpu_ctx):
                 time.sleep(0.0001)
             tickets[i] = srv.submit(pool[i % len(pool)])
 
+    got = []
     try:
         th = threading.Thread(target=submitter)
         th.start()
         for i in range(len(arr)):
             while tickets[i] is None:
                 time.sleep(0.0001)
-            r, tm = srv.wait(tickets[i])
-            assert np.array_equal(r["words"], alone[i % len(pool)]), i
+            got.append(srv.wait(tickets[i]))
         th.join()
         st = srv.stats()
     finally:
         srv.close()
     say("stats", st)
+    check_served(got, [alone[i % len(pool)] for i in range(len(arr))], tickets,
+                 [pool[i % len(pool)].n for i in range(len(arr))])
     assert st["requests_done"] == len(arr) and st["batches"] > 1
 
 
