@@ -25,6 +25,7 @@ enum {
     PTL_ERR_HIP = -5,          /* a HIP runtime call failed */
     PTL_ERR_UNSUPPORTED = -6,  /* input reaches a path the reference leaves undefined */
     PTL_ERR_ADMISSION = -7,    /* serving: request larger than the arena can ever hold */
+    PTL_ERR_TOO_LARGE = -8,    /* ptl_compile_noalloc: input above its node limit */
 };
 
 /* Input: exactly the six DepthTree arrays (pareas-ast include/codegen/depthtree.hpp:12-58,
@@ -206,6 +207,27 @@ int ptl_debug_exclusive_scan(ptl_ctx *ctx, const uint32_t *in, uint32_t *out, ui
  * key patterns; a compile runs it itself. */
 int ptl_debug_depth_sort(ptl_ctx *ctx, const int32_t *keys, const uint32_t *bounds, uint32_t n_programs,
                          uint32_t *out);
+
+/* Compile without allocating.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (a tight serving loop, a harness
+ * or code region that forbids hipMalloc).  Same result as ptl_compile of the
+ * same tree, but the inputs are device arrays (a ptl_ast whose pointers are
+ * device addresses), the outputs go into the caller's device buffers (words:
+ * words_cap entries; the four function-table arrays: funcs_cap entries each),
+ * and every scratch buffer of the pipeline lives in a fixed 128 MiB device
+ * array inside the library, one per GPU, held until the process ends.  The
+ * call allocates no device or pinned host memory.  Inputs above 262144 nodes
+ * are refused (PTL_ERR_TOO_LARGE); if the output does not fit, the call
+ * returns PTL_ERR_ARG with out->n_words / out->n_funcs set to the sizes
+ * needed.  Inputs are checked on the device against the DepthTree contract
+ * (PTL_ERR_INVALID_AST, PTL_ERR_EMPTY as ptl_compile).  On success out holds
+ * the sizes and stage times, and its pointer fields are the caller's device
+ * buffers.  Calls on one GPU share the array: the library serializes them
+ * (per-device lock), and each call leaves its scratch contents behind.
+ * Synchronous; the context's results of an earlier call are dropped. */
+int ptl_compile_noalloc(ptl_ctx *ctx, const ptl_ast *dev_in, uint32_t *words, uint64_t words_cap,
+                        uint32_t *func_ids, uint32_t *func_starts, uint32_t *func_ends, uint32_t *func_sizes,
+                        uint32_t funcs_cap, ptl_result *out);
 
 #ifdef __cplusplus
 }

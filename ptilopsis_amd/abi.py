@@ -8,7 +8,7 @@ STAGES = ("preprocess", "isn_cnt", "isn_gen", "optimize", "regalloc", "fix_jumps
 PTL_OK = 0
 ERRORS = {
     -1: "PTL_ERR_ARG", -2: "PTL_ERR_INVALID_AST", -3: "PTL_ERR_EMPTY", -4: "PTL_ERR_NOMEM",
-    -5: "PTL_ERR_HIP", -6: "PTL_ERR_UNSUPPORTED", -7: "PTL_ERR_ADMISSION",
+    -5: "PTL_ERR_HIP", -6: "PTL_ERR_UNSUPPORTED", -7: "PTL_ERR_ADMISSION", -8: "PTL_ERR_TOO_LARGE",
 }
 
 # every symbol include/ptl.h declares
@@ -18,7 +18,8 @@ EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_uplo
            "ptl_server_release", "ptl_server_stats",
            "ptl_result_asm", "ptl_disassemble", "ptl_tune_set", "ptl_tune_get",
            "ptl_debug_dump", "ptl_debug_stop_after", "ptl_debug_overlap",
-           "ptl_debug_exclusive_scan", "ptl_debug_depth_sort", "ptl_profile_enable", "ptl_profile_get")
+           "ptl_debug_exclusive_scan", "ptl_debug_depth_sort", "ptl_profile_enable", "ptl_profile_get",
+           "ptl_compile_noalloc")
 
 
 class PtlAst(C.Structure):
@@ -102,4 +103,7 @@ def bind(lib):
     lib.ptl_profile_get.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint32)]
     lib.ptl_profile_get.restype = C.c_int
+    lib.ptl_compile_noalloc.argtypes = [C.c_void_p, C.POINTER(PtlAst), C.c_void_p, C.c_uint64] + \
+        [C.c_void_p] * 4 + [C.c_uint32, C.POINTER(PtlResult)]
+    lib.ptl_compile_noalloc.restype = C.c_int
     return lib

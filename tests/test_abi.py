@@ -52,3 +52,5 @@ def test_entry_points_refuse_a_null_context_without_a_device():
     assert lib.ptl_debug_depth_sort(None, keys, bounds, 1, out) == arg
     assert lib.ptl_debug_exclusive_scan(None, out, out, 4) == arg
     assert lib.ptl_debug_overlap(None, 1) == arg
+    r = abi.PtlResult()
+    assert lib.ptl_compile_noalloc(None, None, None, 0, None, None, None, None, 0, C.byref(r)) == arg
