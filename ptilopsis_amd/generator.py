@@ -247,6 +247,27 @@ class Server:
             pass
 
 
+# rvdisasm::color (include/disassembler.hpp:9-17)
+_WHITE, _INSTR, _EXTRA = "\033[37m", "\033[38;5;186m", "\033[38;5;114m"
+
+
+def stage_table(names, ns) -> str:
+    """The stage table rv_generator.cpp:164-202 writes to stderr."""
+    total = sum(ns)
+    name_len = 1 + max(len(n) for n in names)
+    times = [f"{t}ns" for t in ns]
+    total_s = f"{total}ns"
+    time_len = 1 + max(len(total_s), max(len(t) for t in times))
+    pcts = [f"{(t * 100.0) / total:.2f}%" if total else "nan%" for t in ns]
+    pct_len = max(12, 1 + max(len(p) for p in pcts))
+    out = [f"{'Stage':<{name_len}} {'Duration':<{time_len}} % of total\n", "-" * (name_len + time_len + 12) + "\n"]
+    for n, t, p in zip(names, times, pcts):
+        out.append(f"{_INSTR}{n:<{name_len}}{_EXTRA}{t:>{time_len}}{p:>{pct_len}}{_WHITE}\n")
+    out.append("-" * (name_len + time_len + 12) + "\n")
+    out.append(f"{_INSTR}{'Total':<{name_len}}{_EXTRA}{total_s:>{time_len}}{_WHITE}\n")
+    return "".join(out)
+
+
 _EMPTY = np.zeros(0, np.uint32)
 _EMPTY.setflags(write=False)
 
@@ -272,19 +293,16 @@ class RvGeneratorHip:
         self._res = None
 
     def process(self, profile: bool = False) -> None:
-        """Runs the seven stages; prints the stage table to stderr and, with
-        profile, the two-line CSV to stdout (src/codegen/rv_generator.cpp:150-202)."""
+        """Runs the seven stages; with profile, prints the two-line CSV to stdout,
+        then the stage table to stderr, as rv_generator.cpp:150-202 does (stage
+        names, times as C++ chrono prints nanoseconds, shares with two decimals,
+        the disassembler's colour codes)."""
         self._res = self.ctx.compile(self.tree)
         ns = self._res["stage_ns"]
         if profile:
             print(",".join(abi.STAGES) + "," + abi.STAGES[-1])
-            print(",".join(str(x) for x in ns) + "," + str(ns[-1]))
-        total = max(1, sum(ns))
-        w = max(len(s) for s in abi.STAGES) + 1
-        print(f"{'Stage':<{w}} {'Duration':>12} % of total", file=sys.stderr)
-        for name, t in zip(abi.STAGES, ns):
-            print(f"{name:<{w}} {t / 1e6:>9.3f} ms {100.0 * t / total:>9.2f}%", file=sys.stderr)
-        print(f"{'Total':<{w}} {total / 1e6:>9.3f} ms", file=sys.stderr)
+            print(",".join(str(x) for x in ns) + "," + str(ns[-1]), flush=True)
+        sys.stderr.write(stage_table(abi.STAGES, ns))
 
     def get_instructions(self) -> np.ndarray:
         return self._res["words"]
