@@ -1795,6 +1795,239 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
     }
 }
 
+// ============================================================ small programs
+// Programs of at most 64 functions (at most 8 groups of 8) whose words fit
+// the LDS budget are allocated by one wave: lane = function (its index in the
+// program, so lanes 8g .. 8g+7 are group g), each lane keeping the states of
+// its function's virtual registers in LDS, one byte per word (register |
+// touched << 6 | swapped << 7), and its lifetime in registers.  A step is the
+// reference's step (rv_generator_avx.cpp:1041-1418 at one thread) for every
+// function of the program at once:
+//   * a function's state reads see its own group's body's view: the only
+//     effect of the lower groups' bodies in the same step is a mark on a
+//     region-0 entry (only lane-0 functions write region 0, and every lookup
+//     reads it), which an owner sees as a swapped operand (a flip) when it
+//     reads that register.  Owners that read an entry they own are settled in
+//     group order, each from the final lookups of the groups below it, so one
+//     pass reaches the in-order result;
+//   * a lookup of register r marks the virtual that region 0 held at r at the
+//     start of the step.  The owner's commit resets the swapped flag of the
+//     registers it names, so a mark stands if a higher group made it, or if
+//     the owner does not name that register in this step;
+//   * region 0 of the next step: each lane-0 function's rd write, then its rs1
+//     and rs2 frees, groups in order (the last one wins).
+// Several programs can share the wave (program k on lanes seg[k] .. seg[k] +
+// nf - 1, seg[k] a multiple of 8).  Lanes exchange values with DPP inside a
+// group's 8 lanes and with readlane across groups; region 0 of each program
+// is an LDS table read by lane = register, double-buffered by step parity.
+// A register named outside its own function is noted for k_xref_check as in
+// the other kernels; a mark on one, which the owner's bytes cannot hold,
+// sends the program to the exact serial path.
+constexpr uint32_t SM_CHUNK = 32;        // steps per staged chunk of words
+constexpr uint32_t SM_PROGS = 8;         // programs per wave at most
+constexpr uint32_t SM_STATE_OFF = 40960; // LDS bytes before the register states
+constexpr uint32_t SM_STATE_MAX = 114688;  // LDS bytes of register states per wave at most
+constexpr uint32_t SM_STATE_SMALL = 40960; // waves within this need 80 KB of LDS: two per compute unit
+
+struct SmWave { uint32_t np, state_bytes, prog[SM_PROGS], seg[SM_PROGS]; };
+
+// OR of v over the 8 lanes of each group (DPP: swap neighbours, swap pairs,
+// then mirror each half row of 8).
+DEV uint32_t sm_or8(uint32_t v) {
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    return v;
+}
+DEV uint64_t sm_or8_64(uint64_t v) { return (uint64_t)sm_or8((uint32_t)v) | ((uint64_t)sm_or8((uint32_t)(v >> 32)) << 32); }
+DEV uint32_t sm_rl(uint32_t v, uint32_t l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l); }
+DEV uint64_t sm_rl64(uint64_t v, uint32_t l) { return (uint64_t)sm_rl((uint32_t)v, l) | ((uint64_t)sm_rl((uint32_t)(v >> 32), l) << 32); }
+
+DEV uint64_t sm_look(const RaStep &r) {
+    uint64_t m = 0;
+    if (r.swd) m |= 1ull << r.rdr;
+    if (r.sw1) m |= 1ull << r.r1r;
+    if (r.sw2) m |= 1ull << r.r2r;
+    return m;
+}
+
+__global__ __launch_bounds__(64) void k_ra_small(Dev d, const SmWave *waves) {
+    extern __shared__ __align__(16) uint8_t sm_lds[];
+    uint4 (*words)[64] = reinterpret_cast<uint4 (*)[64]>(sm_lds);                        // [SM_CHUNK][64]
+    int32_t (*Sv)[SM_PROGS][64] = reinterpret_cast<int32_t (*)[SM_PROGS][64]>(sm_lds + 32768);  // [2]
+    int8_t (*So)[SM_PROGS][64] = reinterpret_cast<int8_t (*)[SM_PROGS][64]>(sm_lds + 36864);    // [2]
+    uint32_t *slot_base = reinterpret_cast<uint32_t *>(sm_lds + 37888);  // per group slot: lane-0 function's
+    uint32_t *slot_start = slot_base + 8;                                // state base and first word
+    uint32_t *slot_size = slot_base + 16;
+    uint32_t *prog_slots = slot_base + 24;  // per program: first group slot | end slot << 8
+    uint8_t *st = sm_lds + SM_STATE_OFF;
+    __shared__ uint32_t fstart[64], fsize[64];
+    const uint32_t lane = threadIdx.x;
+    const SmWave wv = waves[blockIdx.x];
+    // this lane's program, function and group slot
+    uint32_t pk = 0;
+    for (uint32_t k = 1; k < wv.np; ++k) if (wv.seg[k] <= lane) pk = k;
+    const uint32_t prg = wv.prog[pk], seg = wv.seg[pk];
+    const uint32_t fb = d.pf[prg], nf = d.pf[prg + 1] - fb;
+    const uint32_t li = lane - seg;
+    const bool has = li < nf;
+    const uint32_t f = fb + li;
+    const uint32_t start = has ? d.fstarts[f] : 0u;
+    const uint32_t size = has ? d.fsizes[f] : 0u;
+    const uint32_t slot = lane >> 3, k8 = lane & 7u;
+    // state bytes: exclusive scan of the sizes rounded up to 4
+    const uint32_t sz4 = (size + 3u) & ~3u;
+    uint32_t incl = sz4;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)incl, o, 64);
+        if (lane >= o) incl += u;
+    }
+    const uint32_t base = incl - sz4;
+    uint8_t *my = st + base;
+    for (uint32_t x = 0; x < sz4; x += 4) *reinterpret_cast<uint32_t *>(my + x) = 0u;
+    fstart[lane] = start; fsize[lane] = size;
+    if (k8 == 0) { slot_base[slot] = base; slot_start[slot] = start; slot_size[slot] = size; }
+    if (lane == seg) prog_slots[pk] = (seg >> 3) | (((seg + nf + 7u) >> 3) << 8);
+    for (uint32_t p = 0; p < wv.np; ++p) { Sv[0][p][lane] = -1; So[0][p][lane] = -1; }
+    const uint32_t np = wv.np;
+    uint32_t wmax = size;
+    for (int s = 32; s > 0; s >>= 1) { const uint32_t o = (uint32_t)__shfl_xor((int)wmax, s, 64); wmax = max(wmax, o); }
+    uint64_t life = 0x1Full;
+    uint64_t own = 0;  // lane-0 functions: the region-0 entries this group owns
+    __syncthreads();
+    // state of virtual register v as read by this function: (register, swapped)
+    auto rd_state = [&](uint32_t v, uint32_t &reg, bool &sw) {
+        reg = 0; sw = false;
+        if (!((int32_t)v > 63)) return;
+        if (v - 64u >= d.W0) { atomicOr(d.flags, EF_OOB); return; }
+        const uint32_t x = v - 64u - start;
+        if (x < size) { const uint32_t b = my[x]; reg = b & 0x3Fu; sw = (b >> 7) != 0; }
+        else { const uint32_t g = d.sym[v - 64u]; reg = g & 0xFFu; sw = (g >> 8) != 0; }
+    };
+    auto wr_state = [&](uint32_t v, uint32_t reg) {
+        const uint32_t x = v - 64u - start;
+        if (x < size) my[x] = (uint8_t)(reg | 0x40u);
+        else d.sym[v - 64u] = (uint16_t)reg;
+    };
+    for (uint32_t i = 0; i < wmax; ++i) {
+        if (i % SM_CHUNK == 0) {
+            // thread t stages word i + t % 32 of functions t / 32, t / 32 + 2, ...
+            const uint32_t kk = lane % SM_CHUNK, g0 = lane / SM_CHUNK;
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                uint4 tmp[16];
+#pragma unroll
+                for (uint32_t r = 0; r < 16; ++r) {
+                    const uint32_t g = g0 + 2u * (16u * h + r);
+                    tmp[r] = i + kk < fsize[g] ? d.w0_f4[fstart[g] + i + kk] : make_uint4(0, 0, 0, 0);
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < 16; ++r) words[kk][g0 + 2u * (16u * h + r)] = tmp[r];
+            }
+            __syncthreads();
+        }
+        const uint32_t c = i & 1u, nx = c ^ 1u;
+        const bool act = (int32_t)i < (int32_t)size;
+        const uint4 w = act ? words[i % SM_CHUNK][lane] : make_uint4(0, 0, 0, 0);
+        uint32_t raw1, raw2;
+        bool s1, s2;
+        rd_state(w.y, raw1, s1);
+        rd_state(w.z, raw2, s2);
+        RaStep r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1, raw2, s1, s2, life);
+        if (act) xref_check_fields(d, f, start, size, w.x, w.y, w.z);
+        uint64_t look = act ? sm_look(r) : 0ull;
+        uint64_t gl = sm_or8_64(look);
+        // owners reading an entry they own: settled in group order
+        uint64_t or1 = 0, or2 = 0;
+        if (k8 == 0 && act && own) {
+            uint64_t m = own;
+            while (m) {
+                const uint32_t reg = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const int32_t v = Sv[c][pk][reg];
+                if ((int32_t)w.y > 63 && v == (int32_t)w.y) or1 |= 1ull << reg;
+                if ((int32_t)w.z > 63 && v == (int32_t)w.z) or2 |= 1ull << reg;
+            }
+        }
+        uint64_t cand = __ballot(k8 == 0 && act && (or1 | or2) != 0);
+        while (cand) {
+            const uint32_t L0 = (uint32_t)__builtin_ctzll(cand);
+            cand &= cand - 1;
+            const uint32_t s = L0 >> 3, first = sm_rl(seg, L0) >> 3;
+            uint64_t lower = 0;
+            for (uint32_t t = first; t < s; ++t) lower |= sm_rl64(gl, 8u * t);
+            const bool f1 = lane == L0 && (or1 & lower) != 0, f2 = lane == L0 && (or2 & lower) != 0;
+            if (__ballot(f1 || f2)) {
+                if (f1 || f2) { r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1, raw2, s1 || f1, s2 || f2, life); look = sm_look(r); }
+                gl = sm_or8_64(look);
+            }
+        }
+        // commit: rd, then rs1, then rs2 (a later one wins); the lifetime
+        if (act) {
+            if (((int32_t)r.ord > 63 && r.ord - 64u >= d.W0) || ((int32_t)r.rs1 > 63 && r.rs1 - 64u >= d.W0) ||
+                ((int32_t)r.rs2 > 63 && r.rs2 - 64u >= d.W0)) {
+                atomicOr(d.flags, EF_OOB);
+            } else {
+                if ((int32_t)r.ord > 63) wr_state(r.ord, r.rdr);
+                if ((int32_t)r.rs1 > 63) wr_state(r.rs1, r.r1r);
+                if ((int32_t)r.rs2 > 63) wr_state(r.rs2, r.r2r);
+            }
+            life = r.newlife;
+        }
+        // per program: marks on region 0's virtuals (entry `lane`), then region 0
+        // of the next step.  Entry `lane` needs bit `lane` of the lookups of the
+        // groups below its owner, of its owner's group and of the groups above it.
+        for (uint32_t p = 0; p < np; ++p) {
+            const uint32_t ps = prog_slots[p];
+            const uint32_t s0 = ps & 0xFFu, s1e = ps >> 8;
+            const int32_t v = Sv[c][p][lane];
+            const int32_t o = So[c][p][lane];
+            int32_t nv = v, no = o;
+            bool lowb = false, ownb = false, highb = false, named = false;
+            for (uint32_t t = s0; t < s1e; ++t) {
+                const uint32_t L0 = 8u * t;
+                const bool bt = ((sm_rl64(gl, L0) >> lane) & 1ull) != 0;
+                lowb |= (int32_t)t < o && bt;
+                highb |= (int32_t)t > o && bt;
+                const bool at = sm_rl((uint32_t)act, L0) != 0;
+                const uint32_t ot = sm_rl(w.x, L0), o1 = sm_rl(w.y, L0), o2 = sm_rl(w.z, L0);
+                if ((int32_t)t == o) {
+                    ownb = bt;
+                    named = at && ((uint32_t)v == ot || (uint32_t)v == o1 || (uint32_t)v == o2);
+                }
+                // group t's lane-0 function acts on region 0: rd write, rs1 and rs2 frees
+                if (at) {
+                    const uint32_t rdt = sm_rl(r.rdr, L0), r1t = sm_rl(r.r1r, L0), r2t = sm_rl(r.r2r, L0);
+                    if (ot && rdt == lane) { nv = (int32_t)ot; no = (int32_t)t; }
+                    if (r1t != rdt && r1t == lane) { nv = -1; no = -1; }
+                    if (r2t != rdt && r2t == lane) { nv = -1; no = -1; }
+                }
+            }
+            if (o >= 0 && v >= 64 && (highb || (!named && (lowb || ownb)))) {
+                const uint32_t x = (uint32_t)v - 64u - slot_start[o];
+                if (x < slot_size[o]) st[slot_base[o] + x] |= 0x80u;
+                else { atomicOr(d.flags, EF_XCONFLICT); d.cflag[waves[blockIdx.x].prog[p]] = 1; }
+            }
+            Sv[nx][p][lane] = nv;
+            So[nx][p][lane] = (int8_t)no;
+            for (uint32_t t = s0; t < s1e; ++t) {
+                const uint64_t m = __ballot(no == (int32_t)t);
+                if (lane == 8u * t) own = m;
+            }
+        }
+    }
+    __syncthreads();
+    // final states to HBM (the bytes a function wrote or a mark set), the wave
+    // storing each function's range with consecutive lanes
+    for (uint32_t l = 0; l < 64; ++l) {
+        const uint32_t bl = sm_rl(base, l), sl = sm_rl(start, l), zl = sm_rl(size, l);
+        for (uint32_t x = lane; x < zl; x += 64u) {
+            const uint32_t b = st[bl + x];
+            if (b & 0xC0u) d.sym[sl + x] = (uint16_t)((b & 0x3Fu) | ((b >> 7) << 8));
+        }
+    }
+}
+
 // Lane-0 functions in step lockstep, one workgroup.  Region 0 of the
 // reference's register_state holds the last lane-0 write per physical register;
 // every lookup of any function at step i reads its step-start snapshot S and
@@ -1994,6 +2227,22 @@ DEV void ra0_commit(const Dev &d, const RaStep &r, uint32_t old1, uint32_t old2)
     if (vo && !(v1 && r.rs1 == r.ord) && !(v2 && r.rs2 == r.ord)) d.sym[r.ord - 64u] = (uint16_t)r.rdr;
     if (v1 && !(v2 && r.rs2 == r.rs1) && r.r1r != old1) d.sym[r.rs1 - 64u] = (uint16_t)r.r1r;
     if (v2 && r.r2r != old2) d.sym[r.rs2 - 64u] = (uint16_t)r.r2r;
+}
+
+// The state of virtual register x after this step's commit `r` and marks on the
+// snapshot entries `m`, given its state `g` gathered before them: the commit
+// writes rd, then rs1, then rs2 (a later one wins; ra_commit_syms), and a mark
+// sets the swapped flag afterwards.  Physical operands are left alone.
+DEV void ra0_patch(const RaStep &r, uint64_t m, const int32_t *Sv, uint32_t x, uint32_t &g) {
+    if (!((int32_t)x > 63)) return;
+    if ((int32_t)r.rs2 > 63 && x == r.rs2) g = r.r2r;
+    else if ((int32_t)r.rs1 > 63 && x == r.rs1) g = r.r1r;
+    else if ((int32_t)r.ord > 63 && x == r.ord) g = r.rdr;
+    while (m) {
+        const uint32_t reg = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        if ((uint32_t)Sv[reg] == x) g |= 0x100u;
+    }
 }
 
 // The snapshot registers of `own` that group j owns in snapshot (Sv, So).
@@ -2276,6 +2525,19 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         // vector load (vmcnt 0) here leaves none outstanding across the
         // branches below, so the gathers issue back to back.
         __builtin_amdgcn_s_waitcnt(0x0F70);
+        // The operands' states for step i+1 are gathered first, so that their
+        // latency overlaps this step's commit; only this thread writes them
+        // (its functions' own registers), so the commit and the marks below are
+        // patched into the loaded values afterwards (ra0_patch).
+        uint32_t g1[PER], g2[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            Ra0Fn &f = fn[q];
+            g1[q] = g2[q] = 0;
+            if ((int32_t)(i + 1) < (int32_t)f.size) ra0_gather(d, f.nw.y, f.nw.z, g1[q], g2[q]);
+        }
+        RaStep cm[PER];   // this step's commit per slot (ord 0 and no operands when idle)
+        uint64_t mo[PER];  // snapshot entries of this slot marked at this step
         const uint64_t marked = ((uint64_t)mk[c][1] << 32) | mk[c][0];
         // entries whose region-0 state the commit of step i-1 rewrote; every
         // other entry a group owned at the start of step i-1 it still owns
@@ -2289,12 +2551,14 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             const uint32_t t = tid + q * nth;
             Ra0Fn &f = fn[q];
             const uint32_t j = f.j;
+            cm[q].ord = cm[q].rs1 = cm[q].rs2 = 0; cm[q].r1r = cm[q].r2r = cm[q].rdr = 0;
+            mo[q] = 0;
             if (t >= L) continue;
             uint64_t won = 0;
             if ((int32_t)i < (int32_t)f.size) {
                 ra0_cur(f, cur_rd, cur_rs1, cur_rs2, j);
                 const uint32_t rr = rrs[j];
-                RaStep r;
+                RaStep &r = cm[q];
                 r.ord = f.ord; r.rs1 = f.rs1; r.rs2 = f.rs2;
                 r.r1r = rr & 0xFFu; r.r2r = (rr >> 8) & 0xFFu; r.rdr = (rr >> 16) & 0xFFu;
                 if (r.ord && wkey[c][r.rdr] == (((j + 1u) << 2) | 1u)) {
@@ -2310,6 +2574,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             if (f.own) {
                 const uint64_t o = (f.own & ~kprev) | ra0_owned(f.own & kprev, Sv[c], So[c], j);
                 uint64_t m = o & marked;
+                mo[q] = m;
                 while (m) {
                     const uint32_t reg = (uint32_t)__builtin_ctzll(m);
                     m &= m - 1;
@@ -2321,18 +2586,6 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             }
             f.own |= won;
         }
-        // loads for step i+1: its operands' state (after this step's stores, in
-        // program order) and the fields of step i+2
-        uint32_t g1[PER], g2[PER];
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            Ra0Fn &f = fn[q];
-            g1[q] = g2[q] = 0;
-            if ((int32_t)(i + 1) < (int32_t)f.size) {
-                ra0_gather(d, f.nw.y, f.nw.z, g1[q], g2[q]);
-                g1[q] |= ra0_fpbit(f.nw.w) << 31;
-            }
-        }
         if (reg_rows) {
             if (k0 < copies && i + 2 < maxsize) { smn[k0][lane] = rmn; smx[k0][lane] = rmx; }
         } else if (i + 2 < maxsize) {
@@ -2342,6 +2595,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         // on any path into the next step)
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
+            if ((int32_t)(i + 1) < (int32_t)fn[q].size) {
+                ra0_patch(cm[q], mo[q], Sv[c], fn[q].nw.y, g1[q]);
+                ra0_patch(cm[q], mo[q], Sv[c], fn[q].nw.z, g2[q]);
+                g1[q] |= ra0_fpbit(fn[q].nw.w) << 31;
+            }
             fn[q].pk = ra0_pack(g1[q], g2[q]);
             asm volatile("" : "+v"(fn[q].pk) : : "memory");  // packed here, before the loads below
         }

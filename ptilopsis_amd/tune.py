@@ -35,6 +35,9 @@ KNOBS = {
     "block.emit": ((64, 128, 256), 256, 4),
     "block.fix_jumps": ((64, 128, 256), 256, 5),
     "walk_ring": ((4, 8), 4, 4),
+    # programs of at most 64 functions allocated by one wave each (1) or by the
+    # walk and the lockstep (0)
+    "small_programs": ((0, 1), 1, 4),
 }
 
 _applied = False

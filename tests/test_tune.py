@@ -8,7 +8,7 @@ from ptilopsis_amd import tune
 
 def test_tuning_table_values():
     assert set(tune.KNOBS) == {"block.preprocess", "block.isn_cnt", "block.isn_gen", "block.optimize",
-                               "block.emit", "block.fix_jumps", "walk_ring"}
+                               "block.emit", "block.fix_jumps", "walk_ring", "small_programs"}
     if not os.path.exists(tune.TUNING_PATH):
         return
     t = json.load(open(tune.TUNING_PATH))
