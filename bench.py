@@ -286,7 +286,7 @@ def first_difference(got: dict, want: dict):
     return None
 
 
-def measure_serving(rates, seconds: float, pool_n: int) -> dict:
+def measure_serving(rates, seconds: float, pool_n: int, against_oracle: bool = False) -> dict:
     """Serving stream: Poisson arrivals at each rate for `seconds`, requests
     drawn in order from a pool of programs of log-normal size, submitted to the
     library's request loop (ptl_server) from a host thread as they arrive.  Each
@@ -297,11 +297,25 @@ def measure_serving(rates, seconds: float, pool_n: int) -> dict:
     from ptilopsis_amd import workloads as W
     from ptilopsis_amd.generator import Context, Server
     t0 = time.time()
-    pool = W.serve_pool(pool_n)
-    log(f"serving pool: {pool_n} programs, {sum(t.n for t in pool)} nodes (generated+parsed in {time.time() - t0:.1f}s)")
-    ctx = Context(0)
-    ref = [{k: v for k, v in ctx.compile(t).items() if k in OUT_FIELDS} for t in pool]
-    ctx.close()
+    threads = max(1, min(16, os.cpu_count() or 1) - 1)
+    pool = W.serve_pool(pool_n, threads=threads)
+    log(f"serving pool: {pool_n} programs, {sum(t.n for t in pool)} nodes (generated+parsed in {time.time() - t0:.1f}s "
+        f"on {threads} threads)")
+    t0 = time.time()
+    if against_oracle:
+        # every result is checked against the oracle's output for its program,
+        # computed once per pool program on host threads
+        import oracle
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(threads) as ex:
+            ref = list(ex.map(lambda t: {k: v for k, v in oracle.compile_tree(t).items() if k in OUT_FIELDS}, pool))
+        ref_kind = f"the oracle ({threads} host threads, {time.time() - t0:.1f}s)"
+    else:
+        ctx = Context(0)
+        ref = [{k: v for k, v in ctx.compile(t).items() if k in OUT_FIELDS} for t in pool]
+        ctx.close()
+        ref_kind = "the same program compiled alone"
+    log(f"serving references: {ref_kind}")
     srv = Server(0, hbm_budget=48 << 30, max_batch=4096, workers=2)
     for k in [srv.submit(t) for t in pool[:64]]:  # workers' arenas warm
         srv.wait(k)
@@ -360,7 +374,7 @@ def measure_serving(rates, seconds: float, pool_n: int) -> dict:
         out.append(rec)
     srv.close()
     return {"pool_programs": pool_n, "pool_nodes": sum(t.n for t in pool), "seconds_per_rate": seconds,
-            "workers": 2, "rates": out}
+            "workers": 2, "checked_against": ref_kind, "rates": out}
 
 
 def build_workload(name: str):
@@ -390,6 +404,9 @@ def main() -> int:
     ap.add_argument("--serve-rates", default="500,2000,5000,20000")
     ap.add_argument("--serve-seconds", type=float, default=2.5)
     ap.add_argument("--serve-pool", type=int, default=384)
+    ap.add_argument("--serve-oracle", action="store_true",
+                    help="check every served result against the oracle (computed once per pool program) instead of "
+                         "the same program compiled alone")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy kernel of BYTES bytes three times (to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
@@ -481,7 +498,7 @@ def main() -> int:
     serving = None
     if not args.no_serve:
         serving = measure_serving([float(x) for x in args.serve_rates.split(",")], args.serve_seconds,
-                                  args.serve_pool)
+                                  args.serve_pool, args.serve_oracle)
 
     line = {
         "metric": METRIC, "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,

@@ -100,6 +100,44 @@ int ptl_upload_batch(ptl_ctx *ctx, int slot, const ptl_ast *concat, const int64_
                      const uint32_t *max_depths, uint32_t n_programs);
 int ptl_compile_resident_batch(ptl_ctx *ctx, int slot, int copy_out, ptl_result *out);
 
+/* Parser node records, the input of the device-side tree construction that
+ * replaces DepthTree::DepthTree (pareas-ast src/codegen/depthtree.cpp:11-100):
+ * the tree before flattening, host memory.  Node v (records in any order) has
+ * the children kids[first_child[v] .. first_child[v] + n_children[v]) in child
+ * order; roots[0..n_roots) are the programs' roots (STATEMENT_LIST), in
+ * program order.  Each program becomes exactly the DepthTree that
+ * depthtree.cpp builds from its tree (post-order with children in order, root
+ * depth 0, root parent and child_idx -1), the programs back to back.  Every
+ * record must lie in exactly one root's tree, listed once as a child:
+ * otherwise PTL_ERR_INVALID_AST. */
+typedef struct {
+    const uint8_t *node_types;    /* pareas NodeType */
+    const uint8_t *result_types;  /* pareas DataType */
+    const uint32_t *node_data;
+    const uint32_t *first_child;
+    const uint32_t *n_children;
+    const uint32_t *kids;
+    uint64_t n_nodes;
+    uint64_t n_kids;
+    const uint32_t *roots;
+    uint32_t n_roots;
+} ptl_arena;
+
+/* Builds the DepthTree arrays on the device and copies them into the caller's
+ * arrays (n_nodes entries each, the programs back to back, each program's
+ * parents local to its own node range as ptl_compile_batch takes them);
+ * prog_node_offsets gets n_roots + 1 entries and max_depth the largest depth. */
+int ptl_tree_build(ptl_ctx *ctx, const ptl_arena *in, uint8_t *node_types, uint8_t *result_types, int32_t *parents,
+                   int32_t *depth, int32_t *child_idx, uint32_t *node_data, int64_t *prog_node_offsets,
+                   uint32_t *max_depth);
+/* Builds the trees into resident slot `slot`, as ptl_upload (one root) or
+ * ptl_upload_batch (several) would hold them flattened; no flattened copy
+ * passes through the host. */
+int ptl_upload_arena(ptl_ctx *ctx, int slot, const ptl_arena *in);
+/* Build and compile in one call: out[0..n_roots) as ptl_compile_batch returns
+ * them for the flattened programs (for one root, as ptl_compile). */
+int ptl_compile_arena(ptl_ctx *ctx, const ptl_arena *in, ptl_result *out);
+
 /* Serving (no counterpart in the reference, whose only caller compiles one
  * file per process, src/main.cpp:121-157): a request loop over the batched
  * pipeline.  `workers` threads each own a context (HIP stream + pooled arena)

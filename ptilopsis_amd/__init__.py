@@ -7,4 +7,4 @@ host-side mirror of the reference's ``rv_generator`` interface
 turn Pareas-language source into the generator's input arrays, and the named
 synthetic workloads.
 """
-from .native import host_lib, hip_lib, Tree, parse_source, generate_source  # noqa: F401
+from .native import host_lib, hip_lib, Tree, Arena, parse_source, parse_arena, generate_source  # noqa: F401

@@ -19,7 +19,7 @@ EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_uplo
            "ptl_result_asm", "ptl_disassemble", "ptl_tune_set", "ptl_tune_get",
            "ptl_debug_dump", "ptl_debug_stop_after", "ptl_debug_overlap",
            "ptl_debug_exclusive_scan", "ptl_debug_depth_sort", "ptl_profile_enable", "ptl_profile_get",
-           "ptl_compile_noalloc")
+           "ptl_compile_noalloc", "ptl_tree_build", "ptl_upload_arena", "ptl_compile_arena")
 
 
 class PtlAst(C.Structure):
@@ -27,6 +27,14 @@ class PtlAst(C.Structure):
         ("node_types", C.c_void_p), ("result_types", C.c_void_p), ("parents", C.c_void_p),
         ("depth", C.c_void_p), ("child_idx", C.c_void_p), ("node_data", C.c_void_p),
         ("n_nodes", C.c_uint64), ("max_depth", C.c_uint32),
+    ]
+
+
+class PtlArena(C.Structure):
+    _fields_ = [
+        ("node_types", C.c_void_p), ("result_types", C.c_void_p), ("node_data", C.c_void_p),
+        ("first_child", C.c_void_p), ("n_children", C.c_void_p), ("kids", C.c_void_p),
+        ("n_nodes", C.c_uint64), ("n_kids", C.c_uint64), ("roots", C.c_void_p), ("n_roots", C.c_uint32),
     ]
 
 
@@ -106,4 +114,10 @@ def bind(lib):
     lib.ptl_compile_noalloc.argtypes = [C.c_void_p, C.POINTER(PtlAst), C.c_void_p, C.c_uint64] + \
         [C.c_void_p] * 4 + [C.c_uint32, C.POINTER(PtlResult)]
     lib.ptl_compile_noalloc.restype = C.c_int
+    lib.ptl_tree_build.argtypes = [C.c_void_p, C.POINTER(PtlArena)] + [C.c_void_p] * 8
+    lib.ptl_tree_build.restype = C.c_int
+    lib.ptl_upload_arena.argtypes = [C.c_void_p, C.c_int, C.POINTER(PtlArena)]
+    lib.ptl_upload_arena.restype = C.c_int
+    lib.ptl_compile_arena.argtypes = [C.c_void_p, C.POINTER(PtlArena), C.POINTER(PtlResult)]
+    lib.ptl_compile_arena.restype = C.c_int
     return lib

@@ -534,6 +534,50 @@ extern "C" int ptl_parse_source(const char *src, uint64_t len, ptl_host_tree *ou
     }
 }
 
+extern "C" int ptl_parse_arena(const char *src, uint64_t len, ptl_host_arena *out, char *err, uint64_t err_len) {
+    using namespace ptl_fe;
+    std::memset(out, 0, sizeof(*out));
+    try {
+        Lexer lx(src, (size_t)len);
+        Arena A;
+        A.nodes.reserve(len / 3 + 16);
+        A.kids.reserve(len / 3 + 16);
+        SymTab st;
+        Parser p(lx, A, st);
+        uint32_t root = p.parse();
+        resolve_all(A, root);
+        const uint64_t n = A.nodes.size(), k = A.kids.size();
+        out->node_types = (uint8_t *)malloc(n ? n : 1);
+        out->result_types = (uint8_t *)malloc(n ? n : 1);
+        out->node_data = (uint32_t *)malloc(4 * (n ? n : 1));
+        out->first_child = (uint32_t *)malloc(4 * (n ? n : 1));
+        out->n_children = (uint32_t *)malloc(4 * (n ? n : 1));
+        out->kids = (uint32_t *)malloc(4 * (k ? k : 1));
+        if (!out->node_types || !out->result_types || !out->node_data || !out->first_child || !out->n_children ||
+            !out->kids)
+            throw std::bad_alloc();
+        for (uint64_t v = 0; v < n; ++v) {
+            const Node &x = A.nodes[v];
+            out->node_types[v] = x.type; out->result_types[v] = x.rtype; out->node_data[v] = x.integer;
+            out->first_child[v] = x.first_child; out->n_children[v] = x.nchild;
+        }
+        if (k) std::memcpy(out->kids, A.kids.data(), 4 * k);
+        out->n_nodes = n; out->n_kids = k; out->root = root;
+        out->n_functions = (uint32_t)st.funcs.size();
+        return 0;
+    } catch (const std::exception &e) {
+        ptl_free_host_arena(out);
+        if (err && err_len) { std::strncpy(err, e.what(), err_len - 1); err[err_len - 1] = 0; }
+        return 1;
+    }
+}
+
+extern "C" void ptl_free_host_arena(ptl_host_arena *a) {
+    free(a->node_types); free(a->result_types); free(a->node_data); free(a->first_child); free(a->n_children);
+    free(a->kids);
+    std::memset(a, 0, sizeof(*a));
+}
+
 extern "C" void ptl_free_host_tree(ptl_host_tree *t) {
     free(t->node_types); free(t->result_types); free(t->parents); free(t->depth);
     free(t->child_idx); free(t->node_data);

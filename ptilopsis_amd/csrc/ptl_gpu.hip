@@ -1826,8 +1826,10 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
 constexpr uint32_t SM_CHUNK = 32;        // steps per staged chunk of words
 constexpr uint32_t SM_PROGS = 8;         // programs per wave at most
 constexpr uint32_t SM_STATE_OFF = 40960; // LDS bytes before the register states
-constexpr uint32_t SM_STATE_MAX = 114688;  // LDS bytes of register states per wave at most
-constexpr uint32_t SM_STATE_SMALL = 40960; // waves within this need 80 KB of LDS: two per compute unit
+constexpr uint32_t SM_STATE_MAX = 40960;   // LDS bytes of register states per wave at most (80 KB of
+                                           // LDS in all: two waves per compute unit)
+constexpr uint32_t SM_STATE_SMALL = 40960;
+constexpr uint32_t SM_MIN_FUNCS = 9;       // programs of one group keep k_ra_group
 
 struct SmWave { uint32_t np, state_bytes, prog[SM_PROGS], seg[SM_PROGS]; };
 
@@ -1894,12 +1896,13 @@ __global__ __launch_bounds__(64) void k_ra_small(Dev d, const SmWave *waves) {
     for (int s = 32; s > 0; s >>= 1) { const uint32_t o = (uint32_t)__shfl_xor((int)wmax, s, 64); wmax = max(wmax, o); }
     uint64_t life = 0x1Full;
     uint64_t own = 0;  // lane-0 functions: the region-0 entries this group owns
+    bool bad = false;  // a word names a register past the program's words: this lane stops (EF_OOB)
     __syncthreads();
-    // state of virtual register v as read by this function: (register, swapped)
+    // state of virtual register v as read by this function: (register, swapped);
+    // v is within the words (checked when its chunk was staged)
     auto rd_state = [&](uint32_t v, uint32_t &reg, bool &sw) {
         reg = 0; sw = false;
         if (!((int32_t)v > 63)) return;
-        if (v - 64u >= d.W0) { atomicOr(d.flags, EF_OOB); return; }
         const uint32_t x = v - 64u - start;
         if (x < size) { const uint32_t b = my[x]; reg = b & 0x3Fu; sw = (b >> 7) != 0; }
         else { const uint32_t g = d.sym[v - 64u]; reg = g & 0xFFu; sw = (g >> 8) != 0; }
@@ -1925,16 +1928,26 @@ __global__ __launch_bounds__(64) void k_ra_small(Dev d, const SmWave *waves) {
                 for (uint32_t r = 0; r < 16; ++r) words[kk][g0 + 2u * (16u * h + r)] = tmp[r];
             }
             __syncthreads();
+            // this function's words of the chunk: registers past the words, and
+            // names outside the function (noted for k_xref_check)
+            for (uint32_t k = 0; k < SM_CHUNK && i + k < size; ++k) {
+                const uint4 wk = words[k][lane];
+                if (((int32_t)wk.x > 63 && wk.x - 64u >= d.W0) || ((int32_t)wk.y > 63 && wk.y - 64u >= d.W0) ||
+                    ((int32_t)wk.z > 63 && wk.z - 64u >= d.W0)) {
+                    atomicOr(d.flags, EF_OOB);
+                    bad = true;
+                }
+                xref_check_fields(d, f, start, size, wk.x, wk.y, wk.z);
+            }
         }
         const uint32_t c = i & 1u, nx = c ^ 1u;
-        const bool act = (int32_t)i < (int32_t)size;
+        const bool act = (int32_t)i < (int32_t)size && !bad;
         const uint4 w = act ? words[i % SM_CHUNK][lane] : make_uint4(0, 0, 0, 0);
         uint32_t raw1, raw2;
         bool s1, s2;
         rd_state(w.y, raw1, s1);
         rd_state(w.z, raw2, s2);
         RaStep r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1, raw2, s1, s2, life);
-        if (act) xref_check_fields(d, f, start, size, w.x, w.y, w.z);
         uint64_t look = act ? sm_look(r) : 0ull;
         uint64_t gl = sm_or8_64(look);
         // owners reading an entry they own: settled in group order
@@ -1964,14 +1977,9 @@ __global__ __launch_bounds__(64) void k_ra_small(Dev d, const SmWave *waves) {
         }
         // commit: rd, then rs1, then rs2 (a later one wins); the lifetime
         if (act) {
-            if (((int32_t)r.ord > 63 && r.ord - 64u >= d.W0) || ((int32_t)r.rs1 > 63 && r.rs1 - 64u >= d.W0) ||
-                ((int32_t)r.rs2 > 63 && r.rs2 - 64u >= d.W0)) {
-                atomicOr(d.flags, EF_OOB);
-            } else {
-                if ((int32_t)r.ord > 63) wr_state(r.ord, r.rdr);
-                if ((int32_t)r.rs1 > 63) wr_state(r.rs1, r.r1r);
-                if ((int32_t)r.rs2 > 63) wr_state(r.rs2, r.r2r);
-            }
+            if ((int32_t)r.ord > 63) wr_state(r.ord, r.rdr);
+            if ((int32_t)r.rs1 > 63) wr_state(r.rs1, r.r1r);
+            if ((int32_t)r.rs2 > 63) wr_state(r.rs2, r.r2r);
             life = r.newlife;
         }
         // per program: marks on region 0's virtuals (entry `lane`), then region 0
@@ -2227,22 +2235,6 @@ DEV void ra0_commit(const Dev &d, const RaStep &r, uint32_t old1, uint32_t old2)
     if (vo && !(v1 && r.rs1 == r.ord) && !(v2 && r.rs2 == r.ord)) d.sym[r.ord - 64u] = (uint16_t)r.rdr;
     if (v1 && !(v2 && r.rs2 == r.rs1) && r.r1r != old1) d.sym[r.rs1 - 64u] = (uint16_t)r.r1r;
     if (v2 && r.r2r != old2) d.sym[r.rs2 - 64u] = (uint16_t)r.r2r;
-}
-
-// The state of virtual register x after this step's commit `r` and marks on the
-// snapshot entries `m`, given its state `g` gathered before them: the commit
-// writes rd, then rs1, then rs2 (a later one wins; ra_commit_syms), and a mark
-// sets the swapped flag afterwards.  Physical operands are left alone.
-DEV void ra0_patch(const RaStep &r, uint64_t m, const int32_t *Sv, uint32_t x, uint32_t &g) {
-    if (!((int32_t)x > 63)) return;
-    if ((int32_t)r.rs2 > 63 && x == r.rs2) g = r.r2r;
-    else if ((int32_t)r.rs1 > 63 && x == r.rs1) g = r.r1r;
-    else if ((int32_t)r.ord > 63 && x == r.ord) g = r.rdr;
-    while (m) {
-        const uint32_t reg = (uint32_t)__builtin_ctzll(m);
-        m &= m - 1;
-        if ((uint32_t)Sv[reg] == x) g |= 0x100u;
-    }
 }
 
 // The snapshot registers of `own` that group j owns in snapshot (Sv, So).
@@ -2525,19 +2517,6 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         // vector load (vmcnt 0) here leaves none outstanding across the
         // branches below, so the gathers issue back to back.
         __builtin_amdgcn_s_waitcnt(0x0F70);
-        // The operands' states for step i+1 are gathered first, so that their
-        // latency overlaps this step's commit; only this thread writes them
-        // (its functions' own registers), so the commit and the marks below are
-        // patched into the loaded values afterwards (ra0_patch).
-        uint32_t g1[PER], g2[PER];
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            Ra0Fn &f = fn[q];
-            g1[q] = g2[q] = 0;
-            if ((int32_t)(i + 1) < (int32_t)f.size) ra0_gather(d, f.nw.y, f.nw.z, g1[q], g2[q]);
-        }
-        RaStep cm[PER];   // this step's commit per slot (ord 0 and no operands when idle)
-        uint64_t mo[PER];  // snapshot entries of this slot marked at this step
         const uint64_t marked = ((uint64_t)mk[c][1] << 32) | mk[c][0];
         // entries whose region-0 state the commit of step i-1 rewrote; every
         // other entry a group owned at the start of step i-1 it still owns
@@ -2551,14 +2530,12 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             const uint32_t t = tid + q * nth;
             Ra0Fn &f = fn[q];
             const uint32_t j = f.j;
-            cm[q].ord = cm[q].rs1 = cm[q].rs2 = 0; cm[q].r1r = cm[q].r2r = cm[q].rdr = 0;
-            mo[q] = 0;
             if (t >= L) continue;
             uint64_t won = 0;
             if ((int32_t)i < (int32_t)f.size) {
                 ra0_cur(f, cur_rd, cur_rs1, cur_rs2, j);
                 const uint32_t rr = rrs[j];
-                RaStep &r = cm[q];
+                RaStep r;
                 r.ord = f.ord; r.rs1 = f.rs1; r.rs2 = f.rs2;
                 r.r1r = rr & 0xFFu; r.r2r = (rr >> 8) & 0xFFu; r.rdr = (rr >> 16) & 0xFFu;
                 if (r.ord && wkey[c][r.rdr] == (((j + 1u) << 2) | 1u)) {
@@ -2574,7 +2551,6 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             if (f.own) {
                 const uint64_t o = (f.own & ~kprev) | ra0_owned(f.own & kprev, Sv[c], So[c], j);
                 uint64_t m = o & marked;
-                mo[q] = m;
                 while (m) {
                     const uint32_t reg = (uint32_t)__builtin_ctzll(m);
                     m &= m - 1;
@@ -2586,6 +2562,18 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             }
             f.own |= won;
         }
+        // loads for step i+1: its operands' state (after this step's stores, in
+        // program order) and the fields of step i+2
+        uint32_t g1[PER], g2[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            Ra0Fn &f = fn[q];
+            g1[q] = g2[q] = 0;
+            if ((int32_t)(i + 1) < (int32_t)f.size) {
+                ra0_gather(d, f.nw.y, f.nw.z, g1[q], g2[q]);
+                g1[q] |= ra0_fpbit(f.nw.w) << 31;
+            }
+        }
         if (reg_rows) {
             if (k0 < copies && i + 2 < maxsize) { smn[k0][lane] = rmn; smx[k0][lane] = rmx; }
         } else if (i + 2 < maxsize) {
@@ -2595,11 +2583,6 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         // on any path into the next step)
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            if ((int32_t)(i + 1) < (int32_t)fn[q].size) {
-                ra0_patch(cm[q], mo[q], Sv[c], fn[q].nw.y, g1[q]);
-                ra0_patch(cm[q], mo[q], Sv[c], fn[q].nw.z, g2[q]);
-                g1[q] |= ra0_fpbit(fn[q].nw.w) << 31;
-            }
             fn[q].pk = ra0_pack(g1[q], g2[q]);
             asm volatile("" : "+v"(fn[q].pk) : : "memory");  // packed here, before the loads below
         }

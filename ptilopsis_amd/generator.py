@@ -16,7 +16,7 @@ import sys
 import numpy as np
 
 from . import abi, tune
-from .native import Tree, hip_lib
+from .native import Arena, Tree, hip_lib
 
 
 class PtlError(RuntimeError):
@@ -116,6 +116,54 @@ class Context:
         if s != abi.PTL_OK:
             raise PtlError(s)
         return [result_dict(out[p], copy_out) for p in range(n)] if results else None
+
+    @staticmethod
+    def _arena(ar: Arena):
+        keep = [np.ascontiguousarray(x) for x in (ar.node_types, ar.result_types, ar.node_data, ar.first_child,
+                                                    ar.n_children, ar.kids, ar.roots)]
+        a = abi.PtlArena(keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data, keep[3].ctypes.data,
+                         keep[4].ctypes.data, keep[5].ctypes.data, ar.n, int(ar.kids.shape[0]),
+                         keep[6].ctypes.data, int(ar.roots.shape[0]))
+        return a, keep
+
+    def tree_build(self, ar: Arena) -> list[Tree]:
+        """ptl_tree_build: the DepthTree of every program of the forest, built
+        on the device and copied back."""
+        a, keep = self._arena(ar)
+        n, P = ar.n, int(ar.roots.shape[0])
+        t = np.empty(n, np.uint8); rt = np.empty(n, np.uint8); par = np.empty(n, np.int32)
+        dep = np.empty(n, np.int32); ci = np.empty(n, np.int32); nd = np.empty(n, np.uint32)
+        offs = np.empty(P + 1, np.int64)
+        md = C.c_uint32()
+        s = self.lib.ptl_tree_build(self.h, C.byref(a), t.ctypes.data, rt.ctypes.data, par.ctypes.data,
+                                    dep.ctypes.data, ci.ctypes.data, nd.ctypes.data, offs.ctypes.data, C.byref(md))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        out = []
+        for p in range(P):
+            b, e = int(offs[p]), int(offs[p + 1])
+            out.append(Tree(t[b:e].copy(), rt[b:e].copy(), par[b:e].copy(), dep[b:e].copy(), ci[b:e].copy(),
+                            nd[b:e].copy(), int(dep[b:e].max()), -1))
+        return out
+
+    def upload_arena(self, slot: int, ar: Arena) -> None:
+        """ptl_upload_arena: the trees built on the device into a resident slot."""
+        a, keep = self._arena(ar)
+        s = self.lib.ptl_upload_arena(self.h, slot, C.byref(a))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        self._slot_programs = getattr(self, "_slot_programs", {})
+        self._slot_programs[slot] = int(ar.roots.shape[0])
+
+    def compile_arena(self, ar: Arena, copy_out: bool = True) -> list[dict]:
+        """ptl_compile_arena: build and compile; one result per program."""
+        a, keep = self._arena(ar)
+        P = int(ar.roots.shape[0])
+        out = (abi.PtlResult * P)()
+        s = self.lib.ptl_compile_arena(self.h, C.byref(a), out)
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return [result_dict(out[p], copy_out) for p in range(P)]
 
     def profile(self, on: bool = True) -> None:
         self.lib.ptl_profile_enable(self.h, 1 if on else 0)

@@ -70,6 +70,9 @@ def host_lib():
         lib.ptl_parse_source.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(HostTree), C.c_char_p, C.c_uint64]
         lib.ptl_parse_source.restype = C.c_int
         lib.ptl_free_host_tree.argtypes = [C.POINTER(HostTree)]
+        lib.ptl_parse_arena.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(HostArena), C.c_char_p, C.c_uint64]
+        lib.ptl_parse_arena.restype = C.c_int
+        lib.ptl_free_host_arena.argtypes = [C.POINTER(HostArena)]
         lib.ptl_generate_source.argtypes = [C.POINTER(GenParams), C.POINTER(C.c_uint32), C.POINTER(C.c_char_p),
                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         lib.ptl_generate_source.restype = C.c_int
@@ -137,6 +140,77 @@ def parse_source(src: str | bytes) -> Tree:
     finally:
         lib.ptl_free_host_tree(C.byref(t))
     return tree
+
+
+class HostArena(C.Structure):
+    _fields_ = [
+        ("node_types", C.POINTER(C.c_uint8)),
+        ("result_types", C.POINTER(C.c_uint8)),
+        ("node_data", C.POINTER(C.c_uint32)),
+        ("first_child", C.POINTER(C.c_uint32)),
+        ("n_children", C.POINTER(C.c_uint32)),
+        ("kids", C.POINTER(C.c_uint32)),
+        ("n_nodes", C.c_uint64),
+        ("n_kids", C.c_uint64),
+        ("root", C.c_uint32),
+        ("n_functions", C.c_uint32),
+    ]
+
+
+@dataclass
+class Arena:
+    """Parser node records before flattening: node v has the children
+    kids[first_child[v] : first_child[v] + n_children[v]]; roots are the
+    programs' roots in order (one for a parsed program, several for a forest
+    made by concat)."""
+    node_types: np.ndarray    # u8
+    result_types: np.ndarray  # u8
+    node_data: np.ndarray     # u32
+    first_child: np.ndarray   # u32
+    n_children: np.ndarray    # u32
+    kids: np.ndarray          # u32
+    roots: np.ndarray         # u32
+    n_functions: int = -1
+
+    @property
+    def n(self) -> int:
+        return int(self.node_types.shape[0])
+
+    @staticmethod
+    def concat(arenas) -> "Arena":
+        """One forest of the programs, in order: node and kid indices shifted."""
+        nb = np.cumsum([0] + [a.n for a in arenas])
+        kb = np.cumsum([0] + [int(a.kids.shape[0]) for a in arenas])
+        cat = lambda f: np.concatenate([f(a, i) for i, a in enumerate(arenas)])
+        return Arena(cat(lambda a, i: a.node_types), cat(lambda a, i: a.result_types),
+                     cat(lambda a, i: a.node_data),
+                     cat(lambda a, i: (a.first_child + np.uint32(kb[i])).astype(np.uint32)),
+                     cat(lambda a, i: a.n_children),
+                     cat(lambda a, i: (a.kids + np.uint32(nb[i])).astype(np.uint32)),
+                     cat(lambda a, i: (a.roots + np.uint32(nb[i])).astype(np.uint32)),
+                     sum(a.n_functions for a in arenas))
+
+
+def parse_arena(src: str | bytes) -> Arena:
+    """Front end stopped before flattening (ptl_parse_arena): the typed node
+    records, which flatten to exactly what parse_source returns."""
+    lib = host_lib()
+    b = src.encode() if isinstance(src, str) else src
+    a = HostArena()
+    err = C.create_string_buffer(512)
+    if lib.ptl_parse_arena(b, len(b), C.byref(a), err, 512) != 0:
+        raise ValueError("parse error: " + err.value.decode(errors="replace"))
+    n, k = a.n_nodes, a.n_kids
+    try:
+        def arr(ptr, m, dt):
+            return np.ctypeslib.as_array(ptr, shape=(m,)).astype(dt, copy=True) if m else np.zeros(0, dt)
+        out = Arena(arr(a.node_types, n, np.uint8), arr(a.result_types, n, np.uint8),
+                    arr(a.node_data, n, np.uint32), arr(a.first_child, n, np.uint32),
+                    arr(a.n_children, n, np.uint32), arr(a.kids, k, np.uint32),
+                    np.array([a.root], np.uint32), int(a.n_functions))
+    finally:
+        lib.ptl_free_host_arena(C.byref(a))
+    return out
 
 
 def generate_source(params: dict, targets=None) -> tuple[str, int]:

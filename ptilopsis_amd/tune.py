@@ -37,7 +37,7 @@ KNOBS = {
     "walk_ring": ((4, 8), 4, 4),
     # programs of at most 64 functions allocated by one wave each (1) or by the
     # walk and the lockstep (0)
-    "small_programs": ((0, 1), 1, 4),
+    "small_programs": ((0, 1), 0, 4),
 }
 
 _applied = False

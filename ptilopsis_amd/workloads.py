@@ -75,10 +75,10 @@ def single_50k(seed: int = SEED_SINGLE) -> Tree:
     raise RuntimeError("could not hit the 45k-55k node window")
 
 
-def sized_program(seed: int, total_nodes: int, n_functions: int | None = None, base=BASE,
-                  lo: float = 100.0, hi: float = 20_000.0) -> Tree:
-    """One program whose per-function node targets are log-uniform over [lo, hi]
-    scaled to sum to total_nodes."""
+def sized_source(seed: int, total_nodes: int, n_functions: int | None = None, base=BASE,
+                 lo: float = 100.0, hi: float = 20_000.0) -> str:
+    """Source of one program whose per-function node targets are log-uniform
+    over [lo, hi] scaled to sum to total_nodes."""
     rng = np.random.default_rng(seed)
     if n_functions is None:
         n_functions = max(1, int(round(total_nodes / ((hi - lo) / math.log(hi / lo)))))
@@ -86,11 +86,20 @@ def sized_program(seed: int, total_nodes: int, n_functions: int | None = None, b
     t *= total_nodes / t.sum()
     targets = np.maximum(8, np.round(t)).astype(np.uint32)
     src, _ = program_source(seed, n_functions, targets, base)
-    return parse_source(src)
+    return src
+
+
+def sized_program(seed: int, total_nodes: int, n_functions: int | None = None, base=BASE,
+                  lo: float = 100.0, hi: float = 20_000.0) -> Tree:
+    return parse_source(sized_source(seed, total_nodes, n_functions, base, lo, hi))
+
+
+def large_16m_source(seed: int = SEED_LARGE, total_nodes: int = 16_000_000, n_functions: int = 20_000) -> str:
+    return sized_source(seed, total_nodes, n_functions)
 
 
 def large_16m(seed: int = SEED_LARGE, total_nodes: int = 16_000_000, n_functions: int = 20_000) -> Tree:
-    return sized_program(seed, total_nodes, n_functions)
+    return parse_source(large_16m_source(seed, total_nodes, n_functions))
 
 
 def request_sizes(n: int, seed: int, median: float, sigma: float, lo: int, hi: int) -> np.ndarray:
@@ -98,10 +107,14 @@ def request_sizes(n: int, seed: int, median: float, sigma: float, lo: int, hi: i
     return np.clip(np.round(rng.lognormal(math.log(median), sigma, n)), lo, hi).astype(np.int64)
 
 
-def program_of_size(seed: int, nodes: int, base=BASE) -> Tree:
-    """One program of about `nodes` nodes, functions of ~800 nodes each."""
+def source_of_size(seed: int, nodes: int, base=BASE) -> str:
+    """Source of one program of about `nodes` nodes, functions of ~800 nodes each."""
     nf = max(1, nodes // 800)
-    return sized_program(seed, nodes, nf, base, lo=100.0, hi=4000.0)
+    return sized_source(seed, nodes, nf, base, lo=100.0, hi=4000.0)
+
+
+def program_of_size(seed: int, nodes: int, base=BASE) -> Tree:
+    return parse_source(source_of_size(seed, nodes, base))
 
 
 def batch_4096(n_programs: int = 4096, seed: int = SEED_BATCH) -> list[Tree]:
@@ -127,13 +140,20 @@ def stress_1024(n_programs: int = 1024, seed: int = SEED_STRESS, nodes: int = 64
     return [stress_program((seed << 32) + p, nodes) for p in range(n_programs)]
 
 
-def serve_pool(n: int = 384, seed: int = SEED_REQUESTS) -> list[Tree]:
+def serve_pool(n: int = 384, seed: int = SEED_REQUESTS, threads: int = 1) -> list[Tree]:
     """Programs for the serving stream: request r draws its size from the
     log-normal (median 4k, sigma 2.0, clipped to [200, 4M]) and its program from
     seed SEED_REQUESTS * 2^32 + r.  A stream of more requests than the pool
-    reuses them in order (request r -> program r mod n)."""
+    reuses them in order (request r -> program r mod n).  With threads > 1 the
+    programs are generated on a pool of host threads (the generator and the front
+    end run outside the interpreter lock); the result is the same."""
     sizes = request_sizes(n, seed, 4000, 2.0, 200, 4_000_000)
-    return [program_of_size((seed << 32) + r, int(s)) for r, s in enumerate(sizes)]
+    one = lambda r: program_of_size((seed << 32) + r, int(sizes[r]))
+    if threads <= 1:
+        return [one(r) for r in range(n)]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(one, range(n)))
 
 
 def arrivals(rate: float, seconds: float, seed: int = SEED_ARRIVALS) -> np.ndarray:

@@ -258,39 +258,40 @@ def test_gpu_large_properties(gpu_ctx):
 
 
 def small_program_cases():
+    # (name, tree, whether k_ra_small takes it: 9 to 64 functions, states within 40 KB)
     return [
-        ("one_group", W.program_of_size(0x5EED0B01, 3000)),                       # <= 8 functions
-        ("two_groups", W.stress_program((W.SEED_STRESS << 32) + 19, 64000)),     # 9 functions, a 48k-word one
-        ("64_functions", parse_source(W.program_source(0x5EED0B02, 64, min_stmts=2, max_stmts=6)[0])),
-        ("65_functions", parse_source(W.program_source(0x5EED0B03, 65, min_stmts=2, max_stmts=6)[0])),
-        ("cross_named", W.program_of_size((W.SEED_STRESS << 32) + 35, 12000, W.STRESS)),  # serial path
-        ("large", W.program_of_size((W.SEED_STRESS << 32) + 3, 30000, W.STRESS)),  # states past the LDS budget
-        ("many_floats", parse_source(W.program_source(201, 8, base=dict(W.BASE, float_frac=0.9))[0])),
+        ("one_group", W.program_of_size(0x5EED0B01, 3000), False),                   # k_ra_group
+        ("two_groups", W.program_of_size(0x5EED0B04, 9000), True),
+        ("eight_groups", parse_source(W.program_source(0x5EED0B02, 64, min_stmts=2, max_stmts=6)[0]), True),
+        ("65_functions", parse_source(W.program_source(0x5EED0B03, 65, min_stmts=2, max_stmts=6)[0]), False),
+        ("cross_named", W.program_of_size((W.SEED_STRESS << 32) + 35, 9000, W.STRESS), True),  # serial path
+        ("large_state", W.stress_program((W.SEED_STRESS << 32) + 19, 64000), False),  # 96 KB of states
+        ("stress_small", W.program_of_size((W.SEED_STRESS << 32) + 7, 6000, W.STRESS), False),  # 7 functions
     ]
 
 
 @pytest.mark.parametrize("small", [1, 0])
 def test_gpu_small_programs_both_paths(gpu_ctx, small):
-    # programs of at most 64 functions go to k_ra_small (one wave each) unless
-    # the small_programs knob is 0; both paths give the oracle's words
+    # with the small_programs knob at 1 (default 0), programs of at most 64
+    # functions go to k_ra_small (one wave each); both paths give the oracle's words
     cases = small_program_cases()
     old = [gpu_ctx.tune_get("small_programs", b) for b in range(4)]
     try:
         for b in range(4):
             gpu_ctx.tune_set("small_programs", b, small)
-        for name, t in cases:
+        for name, t, takes in cases:
             say("small programs", small, name, t.n, "nodes")
             gpu_ctx.profile(True)
             got = gpu_ctx.compile(t)
             names = gpu_ctx.kernel_times()
             gpu_ctx.profile(False)
-            assert ("k_ra_small" in names) == (small == 1 and name not in ("65_functions", "large")), (name, sorted(names))
+            assert ("k_ra_small" in names) == (small == 1 and takes), (name, sorted(names))
             if name == "cross_named":
                 assert "k_ra_serial" in names
             assert_same(got, oracle.compile_tree(t))
         say("small programs", small, "batch of all")
-        res = gpu_ctx.compile_batch([t for _, t in cases])
-        for (name, t), r in zip(cases, res):
+        res = gpu_ctx.compile_batch([t for _, t, _ in cases])
+        for (name, t, _), r in zip(cases, res):
             assert_same(r, oracle.compile_tree(t))
     finally:
         for b in range(4):
