@@ -377,13 +377,61 @@ def measure_serving(rates, seconds: float, pool_n: int, against_oracle: bool = F
             "workers": 2, "checked_against": ref_kind, "rates": out}
 
 
-def build_workload(name: str):
+def workload_source(name: str) -> str:
     from ptilopsis_amd import workloads as W
     if name == "large_16m":
-        return W.large_16m()
+        return W.large_16m_source()
     if name == "single_50k":
-        return W.single_50k()
+        for attempt in range(64):
+            src, n = W.program_source(W.SEED_SINGLE + (attempt << 40), 64)
+            if 45_000 <= n <= 55_000:
+                return src
     raise SystemExit(f"unknown workload {name}")
+
+
+TREE_BYTES_PER_NODE = 36  # records read (types 1+1, data 4, first child 4, count 4, kid 4) + six arrays written (18)
+
+
+def measure_tree(ctx, src: str, tree, steps: int, with_cpu: bool) -> dict:
+    """DepthTree construction on the device from the parser's node records
+    (ptl_upload_arena: records copied to HBM, the tour ranked, the six arrays
+    written) against the host front end's flattening of the same records
+    (what depthtree.cpp:11-100 does), and bit-exactness against the arrays the
+    pipeline ran on."""
+    from ptilopsis_amd.native import flatten_records, parse_arena
+    t0 = time.perf_counter()
+    ar = parse_arena(src)
+    t1 = time.perf_counter()
+    ctx.upload_arena(7, ar)
+    ctx.upload_arena(7, ar)
+    w0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.upload_arena(7, ar)
+    w1 = time.perf_counter()
+    ctx.profile(True)
+    ctx.upload_arena(7, ar)
+    kt = ctx.kernel_times()
+    ctx.profile(False)
+    dev_ns = sum(v[0] for v in kt.values())
+    got = ctx.tree_build(ar)[0]
+    bit_exact = all(np.array_equal(a, b) for a, b in zip(got.arrays(), tree.arrays())) and got.max_depth == tree.max_depth
+    alg = TREE_BYTES_PER_NODE * tree.n
+    out = {"nodes": tree.n, "device_ms": dev_ns / 1e6, "wall_ms_with_h2d": (w1 - w0) * 1e3 / steps,
+           "alg_bytes": alg, "achieved_GBps": alg / (dev_ns * 1e-9) / 1e9,
+           "frac": alg / (dev_ns * 1e-9) / 1e9 / HBM_PEAK_GBS,
+           "kernels_ms": {k: v[0] / 1e6 for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])},
+           "host_parse_to_records_ms": (t1 - t0) * 1e3, "bit_exact": bit_exact}
+    if with_cpu:
+        tm = []
+        flatten_records(ar, tm)
+        out["cpu_baseline"] = {"value": tree.n / tm[0], "unit": "nodes/s", "cores": 1, "kind": "port",
+                               "ms": tm[0] * 1e3,
+                               "sample": "the same records flattened by the host front end (ptl_flatten_records, "
+                                         "a restatement of depthtree.cpp:52-100), one thread"}
+    log(f"tree build: {tree.n} nodes, device {out['device_ms']:.3f} ms ({out['achieved_GBps']:.0f} GB/s, "
+        f"frac {out['frac']:.3f}), with H2D of the records {out['wall_ms_with_h2d']:.2f} ms"
+        + (f", host flatten {out['cpu_baseline']['ms']:.1f} ms" if with_cpu else "") + f", bit-exact {bit_exact}")
+    return out
 
 
 def main() -> int:
@@ -428,8 +476,10 @@ def main() -> int:
 
     from ptilopsis_amd.generator import Context
 
+    from ptilopsis_amd import parse_source
     t0 = time.time()
-    tree = build_workload(args.workload)
+    src = workload_source(args.workload)
+    tree = parse_source(src)
     log(f"workload {args.workload}: {tree.n} nodes, {tree.n_functions} functions, depth {tree.max_depth} "
         f"(generated+parsed in {time.time() - t0:.1f}s)")
     ctx = Context(0)
@@ -489,6 +539,9 @@ def main() -> int:
                         ("words", "func_ids", "func_starts", "func_ends", "func_sizes"))
         log(f"cpu oracle: {c1 - c0:.1f}s; bit-exact: {bit_exact}")
 
+    tree_build = measure_tree(ctx, src, tree, 5, not args.no_cpu_baseline)
+    del src
+
     batched = None
     if not args.no_batch:
         batched = [measure_batch(ctx, nm, 1 + k, args.batch_steps, 1, 8,
@@ -508,7 +561,7 @@ def main() -> int:
                    "max_depth": tree.max_depth, "words": res["n_words"], "path": "plain one-program GPU pipeline",
                    "inputs": "resident in HBM"},
         "roofline": roof, "cpu_baseline": cpu, "bit_exact": bit_exact, "stages_ms": stages,
-        "stages_roofline": sroof, "to_asm": to_asm,
+        "stages_roofline": sroof, "to_asm": to_asm, "tree_build": tree_build,
         "batched": batched, "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -460,12 +460,15 @@ static void resolve_all(Arena &A, uint32_t root) {
     }
 }
 
-static void flatten(Arena &A, uint32_t root, ptl_host_tree *out) {
+// Post-order with children in order (depthtree.cpp:52-100), iteratively.  G
+// gives a node's type, result type, integer, child count and i-th child.
+template <class G>
+static void flatten_with(const G &g, uint32_t root, ptl_host_tree *out) {
     uint64_t total = 0;
     {
         std::vector<uint32_t> s{root};
         while (!s.empty()) { uint32_t n = s.back(); s.pop_back(); ++total;
-            for (uint32_t i = 0; i < A.nodes[n].nchild; ++i) s.push_back(A.child(n, i)); }
+            for (uint32_t i = 0; i < g.nchild(n); ++i) s.push_back(g.child(n, i)); }
     }
     out->n_nodes = total;
     out->node_types = (uint8_t *)malloc(total);
@@ -483,9 +486,8 @@ static void flatten(Arena &A, uint32_t root, ptl_host_tree *out) {
     uint64_t i = 0; uint32_t maxd = 0;
     while (!stk.empty()) {
         Fr &f = stk.back();
-        const Node &x = A.nodes[f.n];
-        if (f.next < x.nchild) {
-            uint32_t c = A.child(f.n, f.next);
+        if (f.next < g.nchild(f.n)) {
+            uint32_t c = g.child(f.n, f.next);
             int32_t ci = (int32_t)f.next;
             uint32_t d = f.depth + 1;
             f.next++;
@@ -493,11 +495,11 @@ static void flatten(Arena &A, uint32_t root, ptl_host_tree *out) {
             frame_kids.push_back((uint32_t)kid_pos.size());
             continue;
         }
-        out->node_types[i] = x.type;
-        out->result_types[i] = x.rtype;
+        out->node_types[i] = g.type(f.n);
+        out->result_types[i] = g.rtype(f.n);
         out->depth[i] = (int32_t)f.depth;
         out->child_idx[i] = f.cidx;
-        out->node_data[i] = x.integer;
+        out->node_data[i] = g.integer(f.n);
         out->parents[i] = -1;
         if (f.depth > maxd) maxd = f.depth;
         uint32_t kb = frame_kids.back();
@@ -510,6 +512,26 @@ static void flatten(Arena &A, uint32_t root, ptl_host_tree *out) {
     }
     out->max_depth = maxd;
 }
+
+struct ArenaView {
+    Arena &A;
+    uint8_t type(uint32_t n) const { return A.nodes[n].type; }
+    uint8_t rtype(uint32_t n) const { return A.nodes[n].rtype; }
+    uint32_t integer(uint32_t n) const { return A.nodes[n].integer; }
+    uint32_t nchild(uint32_t n) const { return A.nodes[n].nchild; }
+    uint32_t child(uint32_t n, uint32_t i) const { return A.kids[A.nodes[n].first_child + i]; }
+};
+
+struct RecordView {
+    const ptl_host_arena &a;
+    uint8_t type(uint32_t n) const { return a.node_types[n]; }
+    uint8_t rtype(uint32_t n) const { return a.result_types[n]; }
+    uint32_t integer(uint32_t n) const { return a.node_data[n]; }
+    uint32_t nchild(uint32_t n) const { return a.n_children[n]; }
+    uint32_t child(uint32_t n, uint32_t i) const { return a.kids[a.first_child[n] + i]; }
+};
+
+static void flatten(Arena &A, uint32_t root, ptl_host_tree *out) { flatten_with(ArenaView{A}, root, out); }
 
 }  // namespace ptl_fe
 
@@ -570,6 +592,15 @@ extern "C" int ptl_parse_arena(const char *src, uint64_t len, ptl_host_arena *ou
         if (err && err_len) { std::strncpy(err, e.what(), err_len - 1); err[err_len - 1] = 0; }
         return 1;
     }
+}
+
+extern "C" int ptl_flatten_records(const ptl_host_arena *a, ptl_host_tree *out) {
+    using namespace ptl_fe;
+    std::memset(out, 0, sizeof(*out));
+    if (!a || a->root >= a->n_nodes) return 1;
+    flatten_with(RecordView{*a}, a->root, out);
+    out->n_functions = a->n_functions;
+    return 0;
 }
 
 extern "C" void ptl_free_host_arena(ptl_host_arena *a) {

@@ -40,6 +40,9 @@ typedef struct {
 } ptl_host_arena;
 int ptl_parse_arena(const char *src, uint64_t len, ptl_host_arena *out, char *err, uint64_t err_len);
 void ptl_free_host_arena(ptl_host_arena *a);
+/* The host flattening of records (what ptl_parse_source does after parsing),
+ * one thread; 0 on success.  The records are trusted (no checks). */
+int ptl_flatten_records(const ptl_host_arena *a, ptl_host_tree *out);
 #ifdef __cplusplus
 }
 #endif

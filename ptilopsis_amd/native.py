@@ -73,6 +73,8 @@ def host_lib():
         lib.ptl_parse_arena.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(HostArena), C.c_char_p, C.c_uint64]
         lib.ptl_parse_arena.restype = C.c_int
         lib.ptl_free_host_arena.argtypes = [C.POINTER(HostArena)]
+        lib.ptl_flatten_records.argtypes = [C.POINTER(HostArena), C.POINTER(HostTree)]
+        lib.ptl_flatten_records.restype = C.c_int
         lib.ptl_generate_source.argtypes = [C.POINTER(GenParams), C.POINTER(C.c_uint32), C.POINTER(C.c_char_p),
                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         lib.ptl_generate_source.restype = C.c_int
@@ -211,6 +213,37 @@ def parse_arena(src: str | bytes) -> Arena:
     finally:
         lib.ptl_free_host_arena(C.byref(a))
     return out
+
+
+def _tree_from_host(lib, t: HostTree) -> Tree:
+    n = t.n_nodes
+    try:
+        def arr(ptr, dt):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True) if n else np.zeros(0, dt)
+        return Tree(arr(t.node_types, np.uint8), arr(t.result_types, np.uint8), arr(t.parents, np.int32),
+                    arr(t.depth, np.int32), arr(t.child_idx, np.int32), arr(t.node_data, np.uint32),
+                    int(t.max_depth), int(t.n_functions))
+    finally:
+        lib.ptl_free_host_tree(C.byref(t))
+
+
+def flatten_records(ar: Arena, timing: list | None = None) -> Tree:
+    """The host front end's flattening of one program's records
+    (ptl_flatten_records, one thread); appends its seconds to `timing`."""
+    import time
+    lib = host_lib()
+    keep = [np.ascontiguousarray(x) for x in (ar.node_types, ar.result_types, ar.node_data, ar.first_child,
+                                                ar.n_children, ar.kids)]
+    a = HostArena(keep[0].ctypes.data_as(C.POINTER(C.c_uint8)), keep[1].ctypes.data_as(C.POINTER(C.c_uint8)),
+                  *[k.ctypes.data_as(C.POINTER(C.c_uint32)) for k in keep[2:]], ar.n, int(ar.kids.shape[0]),
+                  int(ar.roots[0]), max(0, ar.n_functions))
+    t = HostTree()
+    t0 = time.perf_counter()
+    if lib.ptl_flatten_records(C.byref(a), C.byref(t)) != 0:
+        raise ValueError("bad records")
+    if timing is not None:
+        timing.append(time.perf_counter() - t0)
+    return _tree_from_host(lib, t)
 
 
 def generate_source(params: dict, targets=None) -> tuple[str, int]:

@@ -98,3 +98,15 @@ def test_concat_shifts_indices():
     of = flatten(f, int(f.roots[1]))
     for k in FIELDS:
         assert np.array_equal(ob[k], of[k]), k
+
+
+@pytest.mark.parametrize("path", FIXTURES[:6], ids=[os.path.basename(p)[:-4] for p in FIXTURES[:6]])
+def test_host_flatten_of_records(path):
+    """ptl_flatten_records (the CPU baseline of the device build) gives the
+    reference fixture's arrays too."""
+    from ptilopsis_amd.native import flatten_records
+    z = np.load(path)
+    t = flatten_records(parse_arena(z["source"].tobytes()))
+    for f in FIELDS:
+        assert np.array_equal(getattr(t, f), z[f]), f
+    assert t.max_depth == int(z["max_depth"][0])
