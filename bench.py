@@ -286,19 +286,22 @@ def first_difference(got: dict, want: dict):
     return None
 
 
-def measure_serving(rates, seconds: float, pool_n: int, against_oracle: bool = False) -> dict:
+def measure_serving(rates, seconds: float, pool_n: int, against_oracle: bool = False, records_rates=()) -> dict:
     """Serving stream: Poisson arrivals at each rate for `seconds`, requests
     drawn in order from a pool of programs of log-normal size, submitted to the
     library's request loop (ptl_server) from a host thread as they arrive.  Each
     result is checked against the same program compiled alone; a mismatch is
     recorded with its ticket, batch, worker, the batch's members and the first
-    differing word."""
+    differing word.  At records_rates the same requests go in as parser records
+    (ptl_server_submit_arena: the tree is built on the device inside the batch,
+    no flattening on the host)."""
     import threading
     from ptilopsis_amd import workloads as W
     from ptilopsis_amd.generator import Context, Server
     t0 = time.time()
     threads = max(1, min(16, os.cpu_count() or 1) - 1)
     pool = W.serve_pool(pool_n, threads=threads)
+    pool_rec = W.serve_pool_records(pool_n, threads=threads) if records_rates else None
     log(f"serving pool: {pool_n} programs, {sum(t.n for t in pool)} nodes (generated+parsed in {time.time() - t0:.1f}s "
         f"on {threads} threads)")
     t0 = time.time()
@@ -319,8 +322,8 @@ def measure_serving(rates, seconds: float, pool_n: int, against_oracle: bool = F
     srv = Server(0, hbm_budget=48 << 30, max_batch=4096, workers=2)
     for k in [srv.submit(t) for t in pool[:64]]:  # workers' arenas warm
         srv.wait(k)
-    out = []
-    for rate in rates:
+
+    def run_rate(rate, records: bool) -> dict:
         arr = W.arrivals(rate, seconds)
         tickets = [None] * len(arr)
 
@@ -329,7 +332,7 @@ def measure_serving(rates, seconds: float, pool_n: int, against_oracle: bool = F
             for i, ta in enumerate(arr):
                 while time.perf_counter() - tb < ta:
                     time.sleep(0.00005)
-                tickets[i] = srv.submit(pool[i % pool_n])
+                tickets[i] = srv.submit_records(pool_rec[i % pool_n]) if records else srv.submit(pool[i % pool_n])
 
         st0 = srv.stats()
         th = threading.Thread(target=submitter)
@@ -360,18 +363,21 @@ def measure_serving(rates, seconds: float, pool_n: int, against_oracle: bool = F
         for d in mism:
             d["batch_members"] = members.get(d["batch_id"], [])
         pct = lambda v, q: float(np.percentile(v, q)) / 1e6
-        rec = {"rate_offered": rate, "requests": len(arr), "seconds": el,
-               "rate_achieved": len(arr) / max(1e-9, float(arr[-1]) if len(arr) else 1.0),
+        rec = {"rate_offered": rate, "input": "records" if records else "arrays", "requests": len(arr),
+               "seconds": el, "rate_achieved": len(arr) / max(1e-9, float(arr[-1]) if len(arr) else 1.0),
                "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "nodes_per_s": nodes / el,
                "batches": nb, "mean_batch": len(arr) / max(1, nb),
                "breakdown_ms": {k[:-3]: {"p50": pct(v, 50), "p99": pct(v, 99)} for k, v in phases.items()},
                "same_as_alone": not mism, "mismatches": mism}
         b = rec["breakdown_ms"]
-        log(f"serving {rate} req/s: p50 {rec['p50_ms']:.2f} ms, p99 {rec['p99_ms']:.2f} ms, "
+        log(f"serving {rate} req/s ({rec['input']}): p50 {rec['p50_ms']:.2f} ms, p99 {rec['p99_ms']:.2f} ms, "
             f"{rec['nodes_per_s'] / 1e6:.1f} M nodes/s, {nb} batches (mean {rec['mean_batch']:.1f}), "
             f"mismatches: {len(mism)}; p50 queue {b['queue']['p50']:.2f} pack {b['pack']['p50']:.2f} "
             f"run {b['run']['p50']:.2f} (device {b['device']['p50']:.2f}) copyout {b['copyout']['p50']:.2f} ms")
-        out.append(rec)
+        return rec
+
+    out = [run_rate(rate, False) for rate in rates]
+    out += [run_rate(rate, True) for rate in records_rates]
     srv.close()
     return {"pool_programs": pool_n, "pool_nodes": sum(t.n for t in pool), "seconds_per_rate": seconds,
             "workers": 2, "checked_against": ref_kind, "rates": out}
@@ -451,6 +457,8 @@ def main() -> int:
                          "(0; for counter collection, which serializes kernels)")
     ap.add_argument("--serve-rates", default="500,2000,5000,20000")
     ap.add_argument("--serve-seconds", type=float, default=2.5)
+    ap.add_argument("--serve-records-rates", default="5000,20000",
+                    help="rates at which the requests are also submitted as parser records (tree built on the device)")
     ap.add_argument("--serve-pool", type=int, default=384)
     ap.add_argument("--serve-oracle", action="store_true",
                     help="check every served result against the oracle (computed once per pool program) instead of "
@@ -551,7 +559,8 @@ def main() -> int:
     serving = None
     if not args.no_serve:
         serving = measure_serving([float(x) for x in args.serve_rates.split(",")], args.serve_seconds,
-                                  args.serve_pool, args.serve_oracle)
+                                  args.serve_pool, args.serve_oracle,
+                                  [float(x) for x in args.serve_records_rates.split(",") if x])
 
     line = {
         "metric": METRIC, "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,

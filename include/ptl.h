@@ -171,6 +171,12 @@ int ptl_server_create(ptl_server **out, int hip_device, uint64_t hbm_budget_byte
                       uint32_t workers);
 void ptl_server_destroy(ptl_server *s);
 int ptl_server_submit(ptl_server *s, const ptl_ast *in, uint64_t *ticket);
+/* A request given as parser records of one program (n_roots == 1; see
+ * ptl_arena): its tree is built on the device inside its batch, so no
+ * flattened copy is made on the host.  Malformed records fail that request
+ * alone (PTL_ERR_INVALID_AST at ptl_server_wait).  The result is the one
+ * ptl_compile gives for the flattened tree. */
+int ptl_server_submit_arena(ptl_server *s, const ptl_arena *in, uint64_t *ticket);
 int ptl_server_wait(ptl_server *s, uint64_t ticket, ptl_result *out, ptl_serve_times *times);
 int ptl_server_release(ptl_server *s, uint64_t ticket);
 int ptl_server_stats(ptl_server *s, uint64_t *batches, uint64_t *requests_done, uint64_t *nodes_done,

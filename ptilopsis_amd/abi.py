@@ -19,7 +19,8 @@ EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_uplo
            "ptl_result_asm", "ptl_disassemble", "ptl_tune_set", "ptl_tune_get",
            "ptl_debug_dump", "ptl_debug_stop_after", "ptl_debug_overlap",
            "ptl_debug_exclusive_scan", "ptl_debug_depth_sort", "ptl_profile_enable", "ptl_profile_get",
-           "ptl_compile_noalloc", "ptl_tree_build", "ptl_upload_arena", "ptl_compile_arena")
+           "ptl_compile_noalloc", "ptl_tree_build", "ptl_upload_arena", "ptl_compile_arena",
+           "ptl_server_submit_arena")
 
 
 class PtlAst(C.Structure):
@@ -120,4 +121,6 @@ def bind(lib):
     lib.ptl_upload_arena.restype = C.c_int
     lib.ptl_compile_arena.argtypes = [C.c_void_p, C.POINTER(PtlArena), C.POINTER(PtlResult)]
     lib.ptl_compile_arena.restype = C.c_int
+    lib.ptl_server_submit_arena.argtypes = [C.c_void_p, C.POINTER(PtlArena), C.POINTER(C.c_uint64)]
+    lib.ptl_server_submit_arena.restype = C.c_int
     return lib

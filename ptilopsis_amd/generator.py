@@ -265,6 +265,16 @@ class Server:
             raise PtlError(s)
         return int(t.value)
 
+    def submit_records(self, ar: Arena) -> int:
+        """ptl_server_submit_arena: one program's parser records; its tree is
+        built on the device inside its batch."""
+        a, keep = Context._arena(ar)
+        t = C.c_uint64()
+        s = self.lib.ptl_server_submit_arena(self.h, C.byref(a), C.byref(t))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return int(t.value)
+
     def wait(self, ticket: int) -> tuple[dict, dict]:
         """(result, times) of a submitted request; its storage is released."""
         r = abi.PtlResult()

@@ -156,6 +156,19 @@ def serve_pool(n: int = 384, seed: int = SEED_REQUESTS, threads: int = 1) -> lis
         return list(ex.map(one, range(n)))
 
 
+def serve_pool_records(n: int = 384, seed: int = SEED_REQUESTS, threads: int = 1):
+    """The same programs as serve_pool(n, seed) as parser records (each
+    flattens to serve_pool's tree of the same index)."""
+    from .native import parse_arena
+    sizes = request_sizes(n, seed, 4000, 2.0, 200, 4_000_000)
+    one = lambda r: parse_arena(source_of_size((seed << 32) + r, int(sizes[r])))
+    if threads <= 1:
+        return [one(r) for r in range(n)]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(one, range(n)))
+
+
 def arrivals(rate: float, seconds: float, seed: int = SEED_ARRIVALS) -> np.ndarray:
     """Poisson arrival times (s) at `rate` requests/s over `seconds`."""
     rng = np.random.default_rng(seed + int(rate))

@@ -130,3 +130,40 @@ def test_gpu_serve_admission():
         assert_same(r, oracle.compile_tree(small))
     finally:
         srv.close()
+
+
+def test_gpu_serve_records_and_arrays_mixed(gpu_ctx):
+    """Requests given as parser records (tree built on the device inside the
+    batch) and as DepthTree arrays share batches; each result equals the alone
+    compile of the flattened tree.  A request with malformed records fails
+    alone: its batch is rerun one request at a time."""
+    from ptilopsis_amd import parse_arena
+    rng = np.random.default_rng(5)
+    sizes = [int(x) for x in np.clip(rng.lognormal(np.log(4000), 1.5, 30), 200, 80_000)]
+    srcs = [W.source_of_size((W.SEED_REQUESTS << 32) + 5000 + i, s) for i, s in enumerate(sizes)]
+    trees = [parse_source(s) for s in srcs]
+    recs = [parse_arena(s) for s in srcs]
+    alone = [gpu_ctx.compile(t) for t in trees]
+    bad = parse_arena(srcs[0])
+    bad.kids = bad.kids.copy()
+    bad.kids[3] = bad.kids[4]  # one node listed twice
+    srv = Server(0, max_batch=64, workers=2)
+    try:
+        # all queued at once, every other one as records: they share batches
+        tickets = [srv.submit_records(recs[i]) if i % 2 == 0 else srv.submit(trees[i]) for i in range(len(srcs))]
+        got = [srv.wait(k) for k in tickets]
+        assert max(tm["batch_programs"] for _, tm in got) > 1
+        check_served(got, alone, tickets, [t.n for t in trees])
+        # the malformed request among good ones
+        some = list(range(0, 12))
+        t2 = [srv.submit_records(recs[i]) for i in some[:6]] + [srv.submit_records(bad)] + \
+             [srv.submit(trees[i]) for i in some[6:]]
+        with pytest.raises(PtlError) as e:
+            srv.wait(t2[6])
+        assert e.value.status == -2
+        got2 = [srv.wait(k) for k in t2[:6] + t2[7:]]
+        check_served(got2, [alone[i] for i in some], t2[:6] + t2[7:], [trees[i].n for i in some])
+        st = srv.stats()
+    finally:
+        srv.close()
+    say("stats", st)
