@@ -220,6 +220,30 @@ def measure_asm(ctx, res: dict, with_cpu: bool) -> dict:
     return out
 
 
+def plain_vs_batched(ctx, trees, slot: int) -> dict:
+    """The same programs through the plain one-program pipeline, one compile
+    each (inputs resident in HBM, words left there), and as one batch."""
+    nodes = sum(t.n for t in trees)
+    plain = 0.0
+    for t in trees:
+        ctx.upload(slot, t)
+        ctx.compile_resident(slot, copy_out=False)
+        a = time.perf_counter()
+        ctx.compile_resident(slot, copy_out=False)
+        plain += time.perf_counter() - a
+    ctx.upload_batch(slot, trees)
+    ctx.compile_resident_batch(slot, copy_out=False, results=False)
+    a = time.perf_counter()
+    ctx.compile_resident_batch(slot, copy_out=False, results=False)
+    batched = time.perf_counter() - a
+    out = {"programs": len(trees), "nodes": nodes, "plain_nodes_per_s": nodes / plain,
+           "batched_nodes_per_s": nodes / batched, "plain_ms": plain * 1e3, "batched_ms": batched * 1e3}
+    log(f"  first {len(trees)} programs ({nodes} nodes): plain pipeline one by one {plain * 1e3:.1f} ms "
+        f"({out['plain_nodes_per_s'] / 1e6:.1f} M nodes/s), one batch {batched * 1e3:.1f} ms "
+        f"({out['batched_nodes_per_s'] / 1e6:.1f} M nodes/s)")
+    return out
+
+
 def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: int, cpu_nodes: int) -> dict:
     """Batched pipeline over one named batch workload, resident in HBM."""
     import oracle
@@ -259,6 +283,7 @@ def measure_batch(ctx, name: str, slot: int, steps: int, warmup: int, n_check: i
            "programs": len(trees), "nodes": nodes, "words": int(sum(r["n_words"] for r in res)),
            "sampled_bit_exact": ok, "sampled_programs": pick, "stages_ms": stages, "stages_roofline": sroof,
            "top_kernels_ms": {k: v[0] / 1e6 for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:6]}}
+    out["plain_vs_batched"] = plain_vs_batched(ctx, trees[:256], slot)
     if cpu_nodes:
         out["cpu_baseline"] = cpu_batch_baseline(trees, res, cpu_nodes)
         log(f"{name}: host baseline {out['cpu_baseline']['value'] / 1e6:.2f} M nodes/s on "

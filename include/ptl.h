@@ -195,6 +195,15 @@ int ptl_server_stats(ptl_server *s, uint64_t *batches, uint64_t *requests_done, 
  * rv_generator.cpp:109-112) are ptl_result.words as they are. */
 int ptl_result_asm(ptl_ctx *ctx, uint32_t program, const char **text, uint64_t *n_bytes);
 int ptl_disassemble(ptl_ctx *ctx, const uint32_t *words, uint64_t n_words, const char **text, uint64_t *n_bytes);
+/* The listing of rv_generator::print(os, true) (src/codegen/rv_generator.cpp:100-103),
+ * i.e. rvdisasm::disassemble(os, words, 0, func_starts) (src/disassembler.cpp:411-451)
+ * of program `program` of the last compile: the header line, then per word its
+ * index, byte address, bytes, mnemonic and coloured operands (branch targets
+ * absolute), with a "func<k>:" line before each function's first word; the
+ * reference's ANSI colours included.  Empty text for a program without words
+ * (the reference prints no listing then).  Text ownership and errors as
+ * ptl_result_asm. */
+int ptl_result_listing(ptl_ctx *ctx, uint32_t program, const char **text, uint64_t *n_bytes);
 
 /* Host-side stage dumps for differential testing (stage 1..7 boundary arrays
  * of the last compile): name is one of "types", "rtypes", "node_data", "locs",

@@ -51,6 +51,8 @@ def lib():
         L.ptlo_introsort.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
         L.ptlo_asm.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
         L.ptlo_asm.restype = C.c_int64
+        L.ptlo_listing.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
+        L.ptlo_listing.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -157,3 +159,15 @@ def ref_front_end(src_path: str, out_path: str):
     from ptilopsis_amd.native import Tree
     return Tree(take(np.uint8, n), take(np.uint8, n), take(np.int32, n), take(np.int32, n), take(np.int32, n),
                 take(np.uint32, n), int(md))
+
+
+def listing(words, func_starts) -> bytes:
+    """rvdisasm::disassemble(os, words, 0, func_starts) text (src/disassembler.cpp:411-451),
+    the listing of rv_generator::print(os, true)."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    fs = np.ascontiguousarray(func_starts, dtype=np.uint32)
+    out = np.zeros(400 * (w.shape[0] + 1) + 256, np.uint8)
+    n = lib().ptlo_listing(w.ctypes.data, w.shape[0], fs.ctypes.data, fs.shape[0], out.ctypes.data, out.shape[0])
+    if n < 0:
+        raise RuntimeError("ptlo_listing: output buffer too small")
+    return out[:n].tobytes()

@@ -28,6 +28,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 #include <stdio.h>
 #include "oracle_tables.h"
 
@@ -1095,37 +1096,48 @@ static int asm_type(uint32_t w) {
     return AT_UNKNOWN;
 }
 
-/* One line: instr_name, ' ', format_args (:224-407, no colour, address 0), '\n'.
- * Returns its length; out needs 64 bytes. */
-static int asm_line(uint32_t w, char *out) {
+/* format_args (:224-407): the operands, with the reference's colour escapes
+ * around registers and immediates when `color`, branch targets relative to
+ * `addr`.  Returns the bytes written (at most 200). */
+#define C_WHITE "\033[37m"
+#define C_INSTR "\033[38;5;186m"
+#define C_REG "\033[38;5;117m"
+#define C_IMM "\033[38;5;114m"
+#define C_EXTRA "\033[38;5;114m"
+static int asm_args(uint32_t w, char *out, int color, uint64_t addr) {
     const uint32_t op = w & 0x7F, f3 = (w >> 12) & 7, f7 = (w >> 25) & 0x7F;
     const uint32_t rd = (w >> 7) & 31, rs1 = (w >> 15) & 31, rs2 = (w >> 20) & 31, rs3 = (w >> 27) & 31;
-    int n = sprintf(out, "%s ", asm_name(w));
-    char *p = out + n;
+    char x[4][40];
+    const char *r0 = color ? C_REG : "", *r1 = color ? C_WHITE : "";
+    const char *i0 = color ? C_IMM : "", *i1 = color ? C_WHITE : "";
+#define XR(k, name) (sprintf(x[k], "%s%s%s", r0, (name), r1), x[k])
+    char *p = out;
     switch (asm_type(w)) {
         case AT_R:
             if (op == 0x53) {
-                if (f7 == 0x78 || f7 == 0x68) p += sprintf(p, "%s, %s", ASM_F[rd], ASM_X[rs1]);
-                else if (f7 == 0x60 || (f7 == 0x70 && f3 == 0)) p += sprintf(p, "%s, %s", ASM_X[rd], ASM_F[rs1]);
-                else p += sprintf(p, "%s, %s, %s", ASM_F[rd], ASM_F[rs1], ASM_F[rs2]);
+                if (f7 == 0x78 || f7 == 0x68) p += sprintf(p, "%s, %s", XR(0, ASM_F[rd]), XR(1, ASM_X[rs1]));
+                else if (f7 == 0x60 || (f7 == 0x70 && f3 == 0)) p += sprintf(p, "%s, %s", XR(0, ASM_X[rd]), XR(1, ASM_F[rs1]));
+                else p += sprintf(p, "%s, %s, %s", XR(0, ASM_F[rd]), XR(1, ASM_F[rs1]), XR(2, ASM_F[rs2]));
             } else {
-                p += sprintf(p, "%s, %s, %s", ASM_X[rd], ASM_X[rs1], ASM_X[rs2]);
+                p += sprintf(p, "%s, %s, %s", XR(0, ASM_X[rd]), XR(1, ASM_X[rs1]), XR(2, ASM_X[rs2]));
             }
             break;
         case AT_I: {
             uint32_t imm = w >> 20;
             if (imm >> 11) imm |= 0xFFFFF000u;
             if (op == 0x07 || op == 0x03 || op == 0x67)
-                p += sprintf(p, "%s, %d(%s)", op == 0x07 ? ASM_F[rd] : ASM_X[rd], (int32_t)imm, ASM_X[rs1]);
+                p += sprintf(p, "%s, %s%d%s(%s)", op == 0x07 ? XR(0, ASM_F[rd]) : XR(0, ASM_X[rd]), i0, (int32_t)imm, i1,
+                             XR(1, ASM_X[rs1]));
             else
-                p += sprintf(p, "%s, %s, %d", ASM_X[rd], ASM_X[rs1], (int32_t)imm);
+                p += sprintf(p, "%s, %s, %s%d%s", XR(0, ASM_X[rd]), XR(1, ASM_X[rs1]), i0, (int32_t)imm, i1);
             break;
         }
         case AT_S: {
             uint32_t imm = ((w >> 7) & 0x1F) | ((w >> 20) & 0xFE0);
             if (imm >> 11) imm |= 0xFFFFF000u;
             /* an fsw names its register by the rs1 field (:357-361) */
-            p += sprintf(p, "%s, %d(%s)", op == 0x27 ? ASM_F[rs1] : ASM_X[rs2], (int32_t)imm, ASM_X[rs1]);
+            p += sprintf(p, "%s, %s%d%s(%s)", op == 0x27 ? XR(0, ASM_F[rs1]) : XR(0, ASM_X[rs2]), i0, (int32_t)imm, i1,
+                         XR(1, ASM_X[rs1]));
             break;
         }
         case AT_B: {
@@ -1133,30 +1145,39 @@ static int asm_line(uint32_t w, char *out) {
              * sign fills bits 31:11 (:366-372) */
             uint32_t imm = ((w >> 7) & 0x1E) | ((w >> 20) & 0x3E0) | (((w >> 7) & 1u) << 11);
             if (w >> 31) imm |= 0xFFFFF800u;
-            const uint64_t tgt = (uint64_t)(int64_t)(int32_t)imm;  /* address 0 + offset, as uint64 */
-            if (tgt) p += sprintf(p, "%s, %s, 0x%llx", ASM_X[rs1], ASM_X[rs2], (unsigned long long)tgt);
-            else p += sprintf(p, "%s, %s, 0", ASM_X[rs1], ASM_X[rs2]);  /* showbase prints 0 bare */
+            const uint64_t tgt = addr + (uint64_t)(int64_t)(int32_t)imm;  /* uint64 address + offset */
+            if (tgt) p += sprintf(p, "%s, %s, %s0x%llx%s", XR(0, ASM_X[rs1]), XR(1, ASM_X[rs2]), i0, (unsigned long long)tgt, i1);
+            else p += sprintf(p, "%s, %s, %s0%s", XR(0, ASM_X[rs1]), XR(1, ASM_X[rs2]), i0, i1);  /* showbase prints 0 bare */
             break;
         }
         case AT_U:
-            p += sprintf(p, "%s, %d", ASM_X[rd], (int32_t)(w & 0xFFFFF000u));
+            p += sprintf(p, "%s, %s%d%s", XR(0, ASM_X[rd]), i0, (int32_t)(w & 0xFFFFF000u), i1);
             break;
         case AT_J: {
             /* bits 10:1, 11, 19:12; the sign fills bits 31:19 (:384-392) */
             uint32_t imm = ((w >> 20) & 0x7FE) | (((w >> 20) & 1u) << 11) | (w & 0xFF000u);
             if (w >> 31) imm |= 0xFFF80000u;
-            p += sprintf(p, "%s, %d", ASM_X[rd], (int32_t)imm);
+            p += sprintf(p, "%s, %s%d%s", XR(0, ASM_X[rd]), i0, (int32_t)imm, i1);
             break;
         }
         case AT_R4:
-            p += sprintf(p, "%s, %s, %s, %s", ASM_X[rd], ASM_X[rs1], ASM_X[rs2], ASM_X[rs3]);
+            p += sprintf(p, "%s, %s, %s, %s", XR(0, ASM_X[rd]), XR(1, ASM_X[rs1]), XR(2, ASM_X[rs2]), XR(3, ASM_X[rs3]));
             break;
         default:
             break;
     }
-    *p++ = '\n';
-    *p = 0;
+#undef XR
     return (int)(p - out);
+}
+
+/* One line: instr_name, ' ', format_args (no colour, address 0), '\n'.
+ * Returns its length; out needs 64 bytes. */
+static int asm_line(uint32_t w, char *out) {
+    int n = sprintf(out, "%s ", asm_name(w));
+    n += asm_args(w, out + n, 0, 0);
+    out[n++] = '\n';
+    out[n] = 0;
+    return n;
 }
 
 /* to_asm of n words into out (cap bytes); returns the text length, or -1 when
@@ -1169,6 +1190,53 @@ int64_t ptlo_asm(const uint32_t *words, int64_t n, char *out, int64_t cap) {
         if (len + k > cap) return -1;
         memcpy(out + len, line, (size_t)k);
         len += k;
+    }
+    return len;
+}
+
+/* rvdisasm::disassemble(os, words, 0, func_starts) (src/disassembler.cpp:411-451),
+ * as rv_generator::print(os, true) calls it (rv_generator.cpp:100-103): a
+ * header line, then per word an optional "func<k>:" line (k counts the labels
+ * printed so far, in hex after word 0; a word gets one when any entry of
+ * func_starts equals its index) and its line: index (decimal, right-aligned, count_digits wide), three
+ * spaces, byte address (hex, zero-filled to digits), ": ", the four bytes high
+ * first (" " and two hex digits each), three spaces, the name left-aligned in
+ * nine columns, a space and the coloured operands.  digits and count_digits as
+ * :416-417 compute them.  Returns the length, or -1 when cap is too small. */
+int64_t ptlo_listing(const uint32_t *words, int64_t n, const uint32_t *func_starts, int64_t nf, char *out,
+                     int64_t cap) {
+    if (n <= 0) return 0;
+    const size_t digits = (size_t)ceil(log((double)n) / log(16.0)) + 1;
+    const size_t count_digits = (size_t)log10((double)n) + 1;
+    int64_t len = 0;
+    char line[512];
+    int k = snprintf(line, sizeof line, C_EXTRA "%lld" C_WHITE " instructions, starting at " C_IMM "0x%0*llx" C_WHITE "\n",
+                     (long long)n, (int)digits, 0ull);
+    if (k > cap) return -1;
+    memcpy(out, line, (size_t)k);
+    len = k;
+    long long func_index = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const uint32_t w = words[i];
+        int m = 0;
+        int is_start = 0;
+        for (int64_t f = 0; f < nf && !is_start; ++f) is_start = func_starts[f] == (uint64_t)i;
+        if (is_start) {
+            m += sprintf(line + m, "%*s", (int)(count_digits + 3 + digits + 2 + 12 + 1), "");
+            /* the stream is still in hex from the previous word's line
+             * (format_args restores the flags it found): only a label before
+             * word 0 is decimal */
+            m += sprintf(line + m, i == 0 ? "func%lld:\n" : "func%llx:\n", func_index++);
+        }
+        m += sprintf(line + m, C_EXTRA "%*lld" C_WHITE "   %0*llx: ", (int)count_digits, (long long)i, (int)digits,
+                     (unsigned long long)(4 * i));
+        for (int b = 3; b >= 0; --b) m += sprintf(line + m, " " C_EXTRA "%02x" C_WHITE, (w >> (8 * b)) & 0xFF);
+        m += sprintf(line + m, "   " C_INSTR "%-9s" C_WHITE " ", asm_name(w));
+        m += asm_args(w, line + m, 1, 4 * (uint64_t)i);
+        line[m++] = '\n';
+        if (len + m > cap) return -1;
+        memcpy(out + len, line, (size_t)m);
+        len += m;
     }
     return len;
 }

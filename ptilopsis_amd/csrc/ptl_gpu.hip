@@ -36,6 +36,7 @@
 // jump targets by the program's word base, so every program's words are
 // exactly those of its own compile, back to back.
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -45,6 +46,7 @@
 #include <vector>
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "rv_spec.h"
 #include "../../include/ptl.h"
@@ -356,6 +358,12 @@ struct CountOp {
     }
     DEV void emit(uint64_t i, uint32_t excl, uint32_t) const { d.loc[i] = excl; }
 };
+
+// The counts alone (the generator's node_sizes member), for the debug print.
+__global__ void k_node_sizes(Dev d, uint32_t *out) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < d.n) out[i] = CountOp{d}.value(i);
+}
 
 // Location fix (:398-447).  A condition child moves its control-flow parent
 // right after its own base words; call arguments then move to their slot in the
@@ -2998,13 +3006,84 @@ template <class S> DEV void asm_hex(S &s, uint64_t v) {
     }
 }
 
-template <class S> DEV void asm_line(S &s, uint32_t w) {
+struct AsmCount {
+    uint32_t n = 0;
+    DEV void put(char) { ++n; }
+};
+
+// rvdisasm::disassemble (src/disassembler.cpp:411-451), one line per word for
+// rv_generator::print(os, true) (src/codegen/rv_generator.cpp:100-103): before
+// a word at a function start a label line (cd + hd + 18 spaces, "func<k>:",
+// k in hex after word 0);
+// then the word index (decimal, right-aligned in cd columns), three spaces, the
+// byte address (hex, zero-padded to hd digits), ": ", the four bytes (hex, high
+// byte first), three spaces, the mnemonic left-aligned in 9 columns, a space
+// and format_args with colour and the word's address (branch targets are
+// absolute).  Colours are the reference's ANSI escapes.
+struct AsmList {
+    uint64_t first;          // word index of the first word formatted
+    const uint32_t *fstarts; // function starts (word indices), sorted, distinct
+    uint32_t nf, cd, hd;     // functions; decimal digits of the word count; hex digits of the addresses
+};
+#define ASM_WHITE "\033[37m"
+#define ASM_INSTR "\033[38;5;186m"
+#define ASM_REG "\033[38;5;117m"
+#define ASM_IMM "\033[38;5;114m"
+#define ASM_EXTRA "\033[38;5;114m"
+template <class S> DEV void asm_cstr(S &s, const char *p) { while (*p) s.put(*p++); }
+// width-padded numbers of the listing's prefix
+template <class S> DEV void asm_udec_w(S &s, uint64_t u, uint32_t width, char fill) {
+    char b[20];
+    int nd = 0;
+    do { b[nd++] = (char)('0' + u % 10u); u /= 10u; } while (u);
+    for (int k = nd; k < (int)width; ++k) s.put(fill);
+    while (nd) s.put(b[--nd]);
+}
+template <class S> DEV void asm_uhex_w(S &s, uint64_t u, uint32_t width) {
+    char b[16];
+    int nd = 0;
+    do { const uint32_t d = (uint32_t)(u & 15u); b[nd++] = (char)(d < 10 ? '0' + d : 'a' + d - 10); u >>= 4; } while (u);
+    for (int k = nd; k < (int)width; ++k) s.put('0');
+    while (nd) s.put(b[--nd]);
+}
+
+template <bool LIST, class S> DEV void asm_line(S &s, uint32_t w, const AsmList &L, uint64_t idx) {
     const uint32_t op = w & 0x7Fu, f3 = (w >> 12) & 7u, f7 = (w >> 25) & 0x7Fu;
     const uint32_t rd = (w >> 7) & 31u, rs1 = (w >> 15) & 31u, rs2 = (w >> 20) & 31u, rs3 = (w >> 27) & 31u;
-    asm_str16(s, c_asm_name[asm_name(w)]);
+    const uint32_t nm = asm_name(w);
+    if (LIST) {
+        // label: the index of this start among the distinct starts
+        uint32_t lo = 0, hi = L.nf;
+        while (lo < hi) { const uint32_t m = (lo + hi) >> 1; if (L.fstarts[m] < idx) lo = m + 1; else hi = m; }
+        if (lo < L.nf && L.fstarts[lo] == idx) {
+            for (uint32_t k = 0; k < L.cd + 3u + L.hd + 2u + 12u + 1u; ++k) s.put(' ');
+            // the stream is still in hex from the previous word's line (format_args
+            // restores the flags it found), so only a label before word 0 is decimal
+            asm_cstr(s, "func");
+            if (idx == 0) asm_udec_w(s, lo, 0, ' '); else asm_uhex_w(s, lo, 0);
+            s.put(':'); s.put('\n');
+        }
+        asm_cstr(s, ASM_EXTRA); asm_udec_w(s, idx, L.cd, ' '); asm_cstr(s, ASM_WHITE);
+        s.put(' '); s.put(' '); s.put(' ');
+        asm_uhex_w(s, 4 * idx, L.hd); s.put(':'); s.put(' ');
+        for (int b = 3; b >= 0; --b) {
+            s.put(' '); asm_cstr(s, ASM_EXTRA); asm_uhex_w(s, (w >> (8 * b)) & 0xFFu, 2); asm_cstr(s, ASM_WHITE);
+        }
+        s.put(' '); s.put(' '); s.put(' ');
+        asm_cstr(s, ASM_INSTR);
+        AsmCount nc;
+        asm_str16(nc, c_asm_name[nm]);
+        asm_str16(s, c_asm_name[nm]);
+        for (uint32_t k = nc.n; k < 9; ++k) s.put(' ');
+        asm_cstr(s, ASM_WHITE);
+    } else {
+        asm_str16(s, c_asm_name[nm]);
+    }
     s.put(' ');
-    auto X = [&](uint32_t r) { asm_str8(s, c_asm_x[r]); };
-    auto F = [&](uint32_t r) { asm_str8(s, c_asm_f[r]); };
+    auto X = [&](uint32_t r) { if (LIST) asm_cstr(s, ASM_REG); asm_str8(s, c_asm_x[r]); if (LIST) asm_cstr(s, ASM_WHITE); };
+    auto F = [&](uint32_t r) { if (LIST) asm_cstr(s, ASM_REG); asm_str8(s, c_asm_f[r]); if (LIST) asm_cstr(s, ASM_WHITE); };
+    auto I0 = [&]() { if (LIST) asm_cstr(s, ASM_IMM); };
+    auto I1 = [&]() { if (LIST) asm_cstr(s, ASM_WHITE); };
     auto sep = [&]() { s.put(','); s.put(' '); };
     switch (op) {
         case 0x33:  // R
@@ -3019,31 +3098,32 @@ template <class S> DEV void asm_line(S &s, uint32_t w) {
             const int32_t imm = (int32_t)w >> 20;
             if (op == 0x07 || op == 0x03 || op == 0x67) {
                 if (op == 0x07) F(rd); else X(rd);
-                sep(); asm_dec(s, imm); s.put('('); X(rs1); s.put(')');
+                sep(); I0(); asm_dec(s, imm); I1(); s.put('('); X(rs1); s.put(')');
             } else {
-                X(rd); sep(); X(rs1); sep(); asm_dec(s, imm);
+                X(rd); sep(); X(rs1); sep(); I0(); asm_dec(s, imm); I1();
             }
             break;
         }
         case 0x23: case 0x27: {  // S; an fsw names its register by the rs1 field (:357-361)
             const int32_t imm = ((int32_t)(w & 0xFE000000u) >> 20) | (int32_t)((w >> 7) & 0x1Fu);
             if (op == 0x27) F(rs1); else X(rs2);
-            sep(); asm_dec(s, imm); s.put('('); X(rs1); s.put(')');
+            sep(); I0(); asm_dec(s, imm); I1(); s.put('('); X(rs1); s.put(')');
             break;
         }
-        case 0x63: {  // B: bits 4:1, 9:5 (not 10) and 11, the sign over bits 31:11 (:366-372); at address 0
+        case 0x63: {  // B: bits 4:1, 9:5 (not 10) and 11, the sign over bits 31:11 (:366-372)
             uint32_t imm = ((w >> 7) & 0x1Eu) | ((w >> 20) & 0x3E0u) | (((w >> 7) & 1u) << 11);
             if (w >> 31) imm |= 0xFFFFF800u;
-            X(rs1); sep(); X(rs2); sep(); asm_hex(s, (uint64_t)(int64_t)(int32_t)imm);
+            const uint64_t addr = LIST ? 4 * idx : 0;
+            X(rs1); sep(); X(rs2); sep(); I0(); asm_hex(s, addr + (uint64_t)(int64_t)(int32_t)imm); I1();
             break;
         }
         case 0x37: case 0x17:  // U
-            X(rd); sep(); asm_dec(s, (int32_t)(w & 0xFFFFF000u));
+            X(rd); sep(); I0(); asm_dec(s, (int32_t)(w & 0xFFFFF000u)); I1();
             break;
         case 0x6F: {  // J: bits 10:1, 11, 19:12, the sign over bits 31:19 (:384-392)
             uint32_t imm = ((w >> 20) & 0x7FEu) | (((w >> 20) & 1u) << 11) | (w & 0xFF000u);
             if (w >> 31) imm |= 0xFFF80000u;
-            X(rd); sep(); asm_dec(s, (int32_t)imm);
+            X(rd); sep(); I0(); asm_dec(s, (int32_t)imm); I1();
             break;
         }
         case 0x43: case 0x47: case 0x4B: case 0x4F:  // R4 (integer names, :395-403)
@@ -3054,11 +3134,6 @@ template <class S> DEV void asm_line(S &s, uint32_t w) {
     }
     s.put('\n');
 }
-
-struct AsmCount {
-    uint32_t n = 0;
-    DEV void put(char) { ++n; }
-};
 
 // Writes bytes from pos on; a 4-byte word lying inside [lo, hi) (the thread's
 // own lines) goes out as one store, the bytes of a word shared with a
@@ -3079,32 +3154,42 @@ struct AsmWrite {
     }
 };
 
-// line lengths: formatted once with a counting sink (the tile sums), kept as
-// one byte per word for the rescan
+// line lengths: formatted once with a counting sink (the tile sums), kept per
+// word for the rescan (one byte for to_asm lines, two for listing lines)
+template <bool LIST>
 struct AsmLenOp {
+    using Len = typename std::conditional<LIST, uint16_t, uint8_t>::type;
     const uint32_t *w;
-    uint8_t *len;
-    DEV uint32_t value(uint64_t i) const { AsmCount c; asm_line(c, w[i]); len[i] = (uint8_t)c.n; return c.n; }
+    Len *len;
+    AsmList L;
+    DEV uint32_t value(uint64_t i) const {
+        AsmCount c;
+        asm_line<LIST>(c, w[i], L, L.first + i);
+        len[i] = (Len)c.n;
+        return c.n;
+    }
     DEV void emit(uint64_t, uint32_t, uint32_t) const {}
 };
+template <class Len>
 struct AsmLenRead {
-    const uint8_t *len;
+    const Len *len;
     DEV uint32_t value(uint64_t i) const { return len[i]; }
 };
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_asm_write(const uint32_t *w, const uint8_t *len8, uint64_t n,
-                                                          const uint32_t *tile_pre, uint8_t *out) {
+template <bool LIST>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_asm_write(const uint32_t *w, const typename AsmLenOp<LIST>::Len *lens,
+                                                          uint64_t n, const uint32_t *tile_pre, uint8_t *out, AsmList L) {
     __shared__ uint32_t s_w[SCAN_BLOCK / 64];
     __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 16];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
-    const uint32_t run = scan_tile(AsmLenRead{len8}, n, tile_pre, s_w, s_v, v);
+    const uint32_t run = scan_tile(AsmLenRead<typename AsmLenOp<LIST>::Len>{lens}, n, tile_pre, s_w, s_v, v);
     uint32_t len = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) len += v[k];
     AsmWrite s{out, run, run, (uint64_t)run + len};
     for (int k = 0; k < SCAN_ITEMS; ++k)
-        if (base + k < n) asm_line(s, w[base + k]);
+        if (base + k < n) asm_line<LIST>(s, w[base + k], L, L.first + base + k);
 }
 
 // ================================================================ host side

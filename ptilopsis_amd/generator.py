@@ -190,6 +190,12 @@ class Context:
         t, n = C.c_void_p(), C.c_uint64()
         return self._text(self.lib.ptl_result_asm(self.h, program, C.byref(t), C.byref(n)), t, n)
 
+    def result_listing(self, program: int = 0) -> bytes:
+        """The listing of rv_generator::print(os, true) for program `program` of
+        the last compile (ptl_result_listing), formatted on the device."""
+        t, n = C.c_void_p(), C.c_uint64()
+        return self._text(self.lib.ptl_result_listing(self.h, program, C.byref(t), C.byref(n)), t, n)
+
     def disassemble(self, words) -> bytes:
         """to_asm text of host words, formatted on the device."""
         w = np.ascontiguousarray(words, dtype=np.uint32)
@@ -342,6 +348,10 @@ def result_dict(r: abi.PtlResult, copy_out: bool = True) -> dict:
                 W1=int(r.n_words_alloc), opt_iterations=int(r.opt_iterations), tie_slots=int(r.tie_slots))
 
 
+# pareas DataType names (pareas-ast src/codegen/datatype.cpp:5-12)
+_TYPE_NAMES = ("invalid", "void", "int", "float", "int_ref", "float_ref")
+
+
 class RvGeneratorHip:
     """Drop-in for rv_generator_avx (rv_generator.hpp:19-74; rv_generator_avx.cpp:104-120)."""
 
@@ -376,6 +386,37 @@ class RvGeneratorHip:
 
     @property
     def function_sizes(self): return self._res["func_sizes"]
+
+    def print(self, stream=None, disassemble: bool = False) -> None:
+        """rv_generator::print(os, disassemble) (rv_generator.cpp:75-107): one line
+        per node of the generator's arrays after process() -- the rv node type,
+        result type name, parent, depth, child index, instruction count, node
+        data and location, each right-aligned as there -- then, with
+        `disassemble` and at least one word, the listing
+        (rvdisasm::disassemble, src/disassembler.cpp:411-451).  The arrays are
+        read back from the device after compiling the tree again (the context's
+        last compile may be another call's)."""
+        stream = stream or sys.stdout
+        ctx = self.ctx
+        self._res = ctx.compile(self.tree)
+        t = self.tree
+        n = t.n
+        ty, rt, nd = ctx.dump("types"), ctx.dump("rtypes"), ctx.dump("node_data")
+        sz, loc = ctx.dump("sizes"), ctx.dump("locs")
+        # digits as the reference computes them (1 + log10 of the array's
+        # largest value, at least 1; log10 of the node count + 1)
+        dig = lambda a: len(str(max(1, int(np.max(a).astype(np.uint32)) if a.size else 1)))
+        nd_d, dep_d, sz_d, dat_d, loc_d = len(str(n)), dig(t.depth), dig(sz), dig(nd), dig(loc)
+        lines = []
+        for i in range(n):
+            lines.append(f"Node {i:>{nd_d}}, type = {int(ty[i]):>3}, result = {_TYPE_NAMES[int(rt[i])]:>9}, "
+                         f"parent = {int(t.parents[i]):>{nd_d}}, depth = {int(t.depth[i]):>{dep_d}}, "
+                         f"child_idx = {int(t.child_idx[i]):>{nd_d + 1}}, size = {int(sz[i]):>{sz_d}}, "
+                         f"data = {int(nd[i]):>{dat_d}}, loc = {int(loc[i]):>{loc_d}}\n")
+        text = "".join(lines)
+        if disassemble and self._res["n_words"]:
+            text += ctx.result_listing(0).decode("ascii")
+        stream.write(text)
 
     def to_binary(self, stream) -> None:
         """Raw little-endian words (rv_generator.cpp:116-119)."""

@@ -49,3 +49,42 @@ def test_gpu_rv_generator_hip_without_profile(gpu_ctx, capsys):
     out, err = capsys.readouterr()
     assert out == "" and "Total" in err
     assert np.array_equal(gen.get_instructions(), oracle.compile_tree(tree)["words"])
+
+
+def node_table(tree, o) -> str:
+    """rv_generator::print's node lines (rv_generator.cpp:75-98) from the
+    oracle's arrays after process()."""
+    names = ("invalid", "void", "int", "float", "int_ref", "float_ref")
+    ty, rt, nd, sz, loc = (o.get(k) for k in ("types", "rtypes", "node_data", "sizes", "locs"))
+    w = lambda a: len(str(max(1, int(a.max()))))
+    n = tree.n
+    nw = len(str(n))
+    return "".join(
+        f"Node {i:>{nw}}, type = {ty[i]:>3}, result = {names[rt[i]]:>9}, parent = {tree.parents[i]:>{nw}}, "
+        f"depth = {tree.depth[i]:>{w(tree.depth)}}, child_idx = {tree.child_idx[i]:>{nw + 1}}, "
+        f"size = {sz[i]:>{w(sz)}}, data = {nd[i]:>{w(nd)}}, loc = {loc[i]:>{w(loc)}}\n" for i in range(n))
+
+
+@pytest.mark.parametrize("which", ["single_50k", "stress"])
+def test_gpu_print_node_table_and_listing(gpu_ctx, which):
+    """print(os, disassemble): the node table from the device's arrays after
+    process() equals the one from the oracle's (node_sizes included), and the
+    listing formatted on the device equals the oracle's restatement of
+    rvdisasm::disassemble (function labels in hex after word 0)."""
+    tree = W.single_50k() if which == "single_50k" else W.program_of_size((W.SEED_STRESS << 32) + 11, 20000, W.STRESS)
+    o = oracle.Oracle(tree)
+    o.run()
+    want = oracle.compile_tree(tree)
+    gen = RvGeneratorHip(tree, gpu_ctx)
+    s = io.StringIO()
+    gen.print(s, disassemble=True)
+    text = s.getvalue()
+    table = node_table(tree, o)
+    assert text.startswith(table)
+    listing = oracle.listing(want["words"], want["func_starts"]).decode()
+    assert text[len(table):] == listing
+    assert np.array_equal(gpu_ctx.dump("sizes"), o.get("sizes"))
+    s2 = io.StringIO()
+    gen.print(s2)
+    assert s2.getvalue() == table
+    o.close()
