@@ -558,6 +558,22 @@ def main() -> int:
 
     to_asm = measure_asm(ctx, res, not args.no_cpu_baseline)
 
+    # kernel-only (sum of the stages' device times) and end-to-end (host arrays
+    # in, words and function table back: ptl_compile) rates of the same compile,
+    # and the device's measured copy bandwidth beside the published HBM peak
+    ctx.compile(tree)
+    e0 = time.perf_counter()
+    ctx.compile(tree)
+    e1 = time.perf_counter()
+    rates = {"kernel_only_nodes_per_s": tree.n / (sum(res["stage_ns"]) * 1e-9),
+             "end_to_end_nodes_per_s": tree.n / (e1 - e0), "end_to_end_ms": (e1 - e0) * 1e3}
+    dev = ctx.copy_bandwidth(1 << 30, 5)
+    roof["measured_copy_GBps"] = dev["GBps"]
+    roof["frac_of_measured_copy"] = (achieved / dev["GBps"]) if achieved else None
+    log(f"kernel-only {rates['kernel_only_nodes_per_s'] / 1e6:.1f} M nodes/s, end-to-end (H2D + compile + D2H) "
+        f"{rates['end_to_end_nodes_per_s'] / 1e6:.1f} M nodes/s; device copy {dev['GBps']:.0f} GB/s "
+        f"({dev['compute_units']} CUs, {dev['hbm_bytes'] / 2**30:.0f} GiB)")
+
     cpu = None
     bit_exact = None
     if not args.no_cpu_baseline:
@@ -595,7 +611,7 @@ def main() -> int:
                    "max_depth": tree.max_depth, "words": res["n_words"], "path": "plain one-program GPU pipeline",
                    "inputs": "resident in HBM"},
         "roofline": roof, "cpu_baseline": cpu, "bit_exact": bit_exact, "stages_ms": stages,
-        "stages_roofline": sroof, "to_asm": to_asm, "tree_build": tree_build,
+        "stages_roofline": sroof, "rates": rates, "device": dev, "to_asm": to_asm, "tree_build": tree_build,
         "batched": batched, "serving": serving,
     }
     print(json.dumps(line), flush=True)

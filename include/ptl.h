@@ -251,6 +251,12 @@ int ptl_profile_get(ptl_ctx *ctx, int idx, const char **name, uint64_t *total_ns
 /* The device exclusive-scan primitive on its own (host arrays in and out),
  * for the known-answer test of benchmarks/prefix_sum.cpp:35-40,215-226. */
 int ptl_debug_exclusive_scan(ptl_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n);
+/* Measured streaming bandwidth of the device: the fastest of `reps` plain copies
+ * of `bytes` bytes (read + write = 2 x bytes moved) in *best_ns, with the device's
+ * compute-unit count and HBM size; for reporting kernels against a measured
+ * peak beside the published one. */
+int ptl_debug_copy_bandwidth(ptl_ctx *ctx, uint64_t bytes, uint32_t reps, uint64_t *best_ns, uint32_t *cus,
+                              uint64_t *hbm_bytes);
 
 /* The device depth sort on its own: each program's node range
  * [bounds[p], bounds[p+1]) of `keys` is ordered as std::sort orders node ids

@@ -20,7 +20,7 @@ EXPORTS = ("ptl_create", "ptl_destroy", "ptl_strerror", "ptl_compile", "ptl_uplo
            "ptl_debug_dump", "ptl_debug_stop_after", "ptl_debug_overlap",
            "ptl_debug_exclusive_scan", "ptl_debug_depth_sort", "ptl_profile_enable", "ptl_profile_get",
            "ptl_compile_noalloc", "ptl_tree_build", "ptl_upload_arena", "ptl_compile_arena",
-           "ptl_server_submit_arena", "ptl_result_listing")
+           "ptl_server_submit_arena", "ptl_result_listing", "ptl_debug_copy_bandwidth")
 
 
 class PtlAst(C.Structure):
@@ -125,4 +125,7 @@ def bind(lib):
     lib.ptl_server_submit_arena.restype = C.c_int
     lib.ptl_result_listing.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
     lib.ptl_result_listing.restype = C.c_int
+    lib.ptl_debug_copy_bandwidth.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
+    lib.ptl_debug_copy_bandwidth.restype = C.c_int
     return lib

@@ -101,6 +101,13 @@ __global__ void k_fill(T *p, uint64_t n, T v) {
 
 __global__ void k_set_u8(uint8_t *p, uint8_t v) { *p = v; }
 
+// A plain streaming copy (16 B per thread and step, grid-stride), for the
+// measured HBM bandwidth the stages are reported against.
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *a, uint4 *b, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 static inline uint32_t grid_for(uint64_t n, uint32_t block) {
     uint64_t g = (n + block - 1) / block;
     if (g == 0) g = 1;

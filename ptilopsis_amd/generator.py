@@ -221,6 +221,17 @@ class Context:
         allocation to be redone one kernel after the other."""
         self.lib.ptl_debug_overlap(self.h, int(on))
 
+    def copy_bandwidth(self, nbytes: int = 1 << 30, reps: int = 5) -> dict:
+        """Measured device copy bandwidth (ptl_debug_copy_bandwidth): GB/s moved
+        (2 x nbytes per copy) by the fastest of `reps` copies, and the device's
+        compute units and HBM size."""
+        ns, cus, hbm = C.c_uint64(), C.c_uint32(), C.c_uint64()
+        s = self.lib.ptl_debug_copy_bandwidth(self.h, nbytes, reps, C.byref(ns), C.byref(cus), C.byref(hbm))
+        if s != abi.PTL_OK:
+            raise PtlError(s)
+        return {"bytes": nbytes, "best_ns": int(ns.value), "GBps": 2 * nbytes / max(1, ns.value),
+                "compute_units": int(cus.value), "hbm_bytes": int(hbm.value)}
+
     def exclusive_scan(self, x: np.ndarray) -> np.ndarray:
         a = np.ascontiguousarray(x, dtype=np.uint32)
         out = np.zeros_like(a)
