@@ -51,6 +51,10 @@ def lib():
         L.ptlo_introsort.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
         L.ptlo_asm.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
         L.ptlo_asm.restype = C.c_int64
+        L.ptlo_run_scalar.argtypes = [C.c_void_p, C.c_int, C.c_uint32]
+        L.ptlo_run_scalar.restype = C.c_uint32
+        L.ptlo_run_stage.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint32]
+        L.ptlo_run_stage.restype = C.c_uint32
         L.ptlo_listing.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
         L.ptlo_listing.restype = C.c_int64
         _lib = L
@@ -69,6 +73,18 @@ class Oracle:
 
     def run(self, upto: int = 7) -> int:
         self.flags = lib().ptlo_run(self.h, upto)
+        return self.flags
+
+    def run_scalar(self, upto: int = 7, avx: int = 0) -> int:
+        """The scalar generator's semantics instead (rv_generator_st,
+        src/codegen/rv_generator.cpp:227-1736), with the AVX2 behaviours named
+        by the SC_* bits in `avx`; a context runs its stages once."""
+        self.flags = lib().ptlo_run_scalar(self.h, upto, avx)
+        return self.flags
+
+    def run_stage(self, stage: int, scalar: bool = False, avx: int = 0) -> int:
+        """The next stage alone, of either semantics (see ptlo_run_stage)."""
+        self.flags = lib().ptlo_run_stage(self.h, stage, 1 if scalar else 0, avx)
         return self.flags
 
     def get(self, name: str) -> np.ndarray:
@@ -107,9 +123,17 @@ class Oracle:
             pass
 
 
-def compile_tree(tree) -> dict:
+# SC_* bits of oracle/ptl_scalar.inc: divergences switched to the AVX2 behaviour
+SC = {"D1": 1 << 1, "D2": 1 << 2, "D3": 1 << 3, "D4": 1 << 4, "D6": 1 << 6, "D9": 1 << 9, "D10": 1 << 10,
+      "D11": 1 << 11, "D12": 1 << 12, "D13": 1 << 13, "D14": 1 << 14, "D15": 1 << 15, "D17": 1 << 17}
+SC_ALL = sum(SC.values())
+
+
+def compile_tree(tree, scalar: bool = False, avx: int = 0) -> dict:
+    """The oracle's words and function table for a tree: rv_generator_avx at one
+    thread, or with scalar=True rv_generator_st (with the AVX2 behaviours in `avx`)."""
     o = Oracle(tree)
-    flags = o.run(7)
+    flags = o.run_scalar(7, avx) if scalar else o.run(7)
     r = o.result()
     r["flags"] = flags
     o.close()

@@ -1240,3 +1240,5 @@ int64_t ptlo_listing(const uint32_t *words, int64_t n, const uint32_t *func_star
     }
     return len;
 }
+
+#include "ptl_scalar.inc"
