@@ -2316,21 +2316,22 @@ DEV void ra0_wait(const Dev &d, const uint32_t *done, const uint32_t *need, uint
     }
 }
 
-// Stages row r of the walk's tables in LDS (smn/smx[copy]) for wave 0 to fold
-// a step later: copy k is loaded by wave nwv-1-(k mod nwv), the last waves
-// holding the shortest functions (idle through most steps), so that no wave
-// with work waits for rows coming from HBM.  A loading wave waits for the
-// row's chunk to be published (`first`: for the chunk of a first row).
+// Folds row r of the walk's tables into the lockstep's lookup row of its step
+// (mn / mx, cleared beforehand; min / max over the copies by LDS atomics):
+// copy k is loaded by wave nwv-1-(k mod nwv), the last waves holding the
+// shortest functions (idle through most steps), so that no wave with work
+// waits for rows coming from HBM.  A loading wave waits for the row's chunk to
+// be published (`first`: for the chunk of a first row).
 DEV void ra0_stage(const Dev &d, const uint32_t *gmin, const uint32_t *gmax, uint64_t stride, uint32_t copies,
                    uint32_t r, const uint32_t *pdone, const uint32_t *pneed, bool first, bool &ok,
-                   uint32_t (*smn)[64], uint32_t (*smx)[64], uint32_t wid, uint32_t nwv, uint32_t lane) {
+                   uint32_t *mn, uint32_t *mx, uint32_t wid, uint32_t nwv, uint32_t lane) {
     const uint32_t k0 = nwv - 1u - wid;
     if (k0 >= copies) return;
     if (pdone && (first || r % WALK_PUB == 0)) ra0_wait(d, pdone, pneed, r / WALK_PUB, ok);
     for (uint32_t k = k0; k < copies; k += nwv) {
         const uint64_t o = (uint64_t)k * stride + (uint64_t)r * 64u + lane;
-        smn[k][lane] = gmin[o];
-        smx[k][lane] = gmax[o];
+        atomicMin(&mn[lane], gmin[o]);
+        atomicMax(&mx[lane], gmax[o]);
     }
 }
 
@@ -2342,13 +2343,16 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
     __shared__ uint32_t cur_rd[NL], cur_rs1[NL], cur_rs2[NL];
     __shared__ uint32_t rrs[NL];                              // step result per group: r1r | r2r << 8 | rdr << 16
     __shared__ int32_t Sv[2][64], So[2][64];                  // region-0 snapshot of step i / i+1
-    __shared__ uint32_t Lmin[2][64], Lmax[2][64];             // lookups by all groups but the owners
+    // lookups by all groups but the owners, and by the walk: the walk's row of
+    // a step is folded in two steps ahead, into the table cleared after its last
+    // read
+    __shared__ uint32_t Lmin[2][64], Lmax[2][64];
     __shared__ uint32_t Omin[2][64], Omax[2][64];             // lookups by the (possible) owners
     __shared__ uint32_t wkey[2][64];
     __shared__ uint32_t flip[64];                             // flip bits per snapshot entry
+    __shared__ uint32_t fmk[2];                               // snapshot entries with flip bits
     __shared__ uint32_t mk[2][2];                             // snapshot entries marked at step i
     __shared__ uint32_t keyed[2][2];                          // snapshot entries rewritten in the commit of step i
-    __shared__ uint32_t smn[RA0_MAXCOPIES][64], smx[RA0_MAXCOPIES][64];  // staged row of step i+1, per copy
     __shared__ int changed;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const uint32_t wid = tid >> 6, nwv = nth >> 6, lane = tid & 63u;
@@ -2398,9 +2402,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             ra0_row(gmin, gmax, stride, copies, 0, tid, m0, x0);
         }
         Lmin[0][tid] = m0; Lmax[0][tid] = x0;
+        Lmin[1][tid] = 0xFFFFFFFFu; Lmax[1][tid] = 0u;
         Omin[0][tid] = Omin[1][tid] = 0xFFFFFFFFu; Omax[0][tid] = Omax[1][tid] = 0;
     }
-    if (maxsize > 1) ra0_stage(d, gmin, gmax, stride, copies, 1, pdone, pneed, true, wok, smn, smx, wid, nwv, lane);
+    __syncthreads();
+    if (maxsize > 1) ra0_stage(d, gmin, gmax, stride, copies, 1, pdone, pneed, true, wok, Lmin[1], Lmax[1], wid, nwv, lane);
     if (reg_rows && k0 < copies && maxsize > 2) {
         if (pdone && 2u % WALK_PUB == 0) ra0_wait(d, pdone, pneed, 2u / WALK_PUB, wok);
         const uint64_t o = (uint64_t)k0 * stride + 2u * 64u + lane;
@@ -2448,12 +2454,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             flip[tid] = nb;
             fb = nb;
             const uint64_t any = __ballot(nb != 0);
-            if (tid == 0) changed = any != 0;
-            // the walk's lookups of step i+1 (staged at the end of step i-1)
-            uint32_t mn = 0xFFFFFFFFu, mx = 0;
-            if (i + 1 < maxsize)
-                for (uint32_t k = 0; k < copies; ++k) { mn = min(mn, smn[k][tid]); mx = max(mx, smx[k][tid]); }
-            Lmin[n][tid] = mn; Lmax[n][tid] = mx;
+            if (tid == 0) { changed = any != 0; fmk[0] = (uint32_t)any; fmk[1] = (uint32_t)(any >> 32); }
         }
         if (tid >= cb)
             for (uint32_t k = tid - cb; k < 64; k += nth - cb) wkey[n][k] = 0;
@@ -2463,12 +2464,14 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             // flip bits are stable (lower groups settle first, so this reaches
             // the in-order result); O[c] was cleared when wave 0 last read it
             for (uint32_t round = 0;; ++round) {
+                // only entries with flip bits can give an owner flips
+                const uint64_t fm = ((uint64_t)fmk[1] << 32) | fmk[0];
 #pragma unroll
                 for (int q = 0; q < PER; ++q) {
                     const uint32_t j = fn[q].j;
                     if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
                         fn[q].pk = (fn[q].pk & 0x8003FFFFu) |
-                                   (ra0_flips(ra0_owned(fn[q].own, Sv[c], So[c], j), flip) << 18);
+                                   (ra0_flips(ra0_owned(fn[q].own & fm, Sv[c], So[c], j), flip) << 18);
                         ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
                         const RaStep r = ra0_step(fn[q]);
                         ra0_lookups(Omin[c], Omax[c], r, j);
@@ -2486,9 +2489,9 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                         if (cur_rs2[o] == (uint32_t)v) nb |= 2u;
                     }
                     flip[tid] = nb;
-                    const uint64_t diff = __ballot(nb != fb);
+                    const uint64_t diff = __ballot(nb != fb), any = __ballot(nb != 0);
                     fb = nb;
-                    if (tid == 0) changed = diff != 0;
+                    if (tid == 0) { changed = diff != 0; fmk[0] = (uint32_t)any; fmk[1] = (uint32_t)(any >> 32); }
                 }
                 lds_barrier();
                 if (!changed) break;
@@ -2516,6 +2519,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             const int32_t v = Sv[c][tid], o = So[c][tid];
             Sv[n][tid] = v; So[n][tid] = o;
             const uint32_t mx = max(Lmax[c][tid], ox);
+            Lmin[c][tid] = 0xFFFFFFFFu; Lmax[c][tid] = 0u;  // last read: cleared for the walk's row of step i+2
             bool mark = false;
             if (v >= 64 && o >= 0 && mx != 0) {
                 mark = true;
@@ -2590,9 +2594,9 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             }
         }
         if (reg_rows) {
-            if (k0 < copies && i + 2 < maxsize) { smn[k0][lane] = rmn; smx[k0][lane] = rmx; }
+            if (k0 < copies && i + 2 < maxsize) { atomicMin(&Lmin[c][lane], rmn); atomicMax(&Lmax[c][lane], rmx); }
         } else if (i + 2 < maxsize) {
-            ra0_stage(d, gmin, gmax, stride, copies, i + 2, pdone, pneed, false, wok, smn, smx, wid, nwv, lane);
+            ra0_stage(d, gmin, gmax, stride, copies, i + 2, pdone, pneed, false, wok, Lmin[c], Lmax[c], wid, nwv, lane);
         }
         // (packed for every slot, finished ones too: no gather stays pending
         // on any path into the next step)
