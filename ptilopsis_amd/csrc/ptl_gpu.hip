@@ -1472,6 +1472,16 @@ constexpr uint64_t RA0_WAIT_TICKS = 25000000ull;  // 0.25 s
 struct Ra0Prog { uint32_t fbase, L, maxsize, poff; uint64_t gbase; uint32_t nf, rows, cbase, pad; };
 constexpr uint32_t RA0_NOPUB = 0xFFFFFFFFu;
 
+// OR of v over the 8 lanes of each group of a wave, every lane active (DPP:
+// swap neighbours, swap pairs, then mirror each half row of 8).
+DEV uint32_t sm_or8(uint32_t v) {
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    return v;
+}
+DEV uint64_t sm_or8_64(uint64_t v) { return (uint64_t)sm_or8((uint32_t)v) | ((uint64_t)sm_or8((uint32_t)(v >> 32)) << 32); }
+
 // The walk's cache of a function's states, kept by one thread: ring slot
 // x & WALK_RING holds the newest definitions (tag x << 10 | valid << 9 |
 // state); the LDS cache c[x % WALK_CACHE][lane] holds older ones; HBM (sym)
@@ -1577,6 +1587,7 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
     uint32_t wmax = size;
     for (int s = 32; s > 0; s >>= 1) { uint32_t o = __shfl_xor(wmax, s, 64); wmax = wmax > o ? wmax : o; }
     uint64_t life = 0x1Full;
+    uint4 wn = make_uint4(0, 0, 0, 0);  // the word of the next step
     __syncthreads();
     for (uint32_t i = 0; i < wmax; ++i) {
         if (i % WALK_CHUNK == 0) {
@@ -1594,9 +1605,11 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
                 for (uint32_t r = 0; r < 16; ++r) words[k][g0 + 2u * (16u * h + r)] = tmp[r];
             }
             __syncthreads();
+            wn = words[0][lane];
         }
+        const uint4 w = wn;
+        if ((i + 1) % WALK_CHUNK != 0) wn = words[(i + 1) % WALK_CHUNK][lane];  // a step ahead
         if ((int32_t)i < (int32_t)size) {
-            const uint4 w = words[i % WALK_CHUNK][lane];
             const uint32_t raw1 = virt_ok(d, w.y) ? (get(w.y) & 0xFFu) : 0u;
             const uint32_t raw2 = virt_ok(d, w.z) ? (get(w.z) & 0xFFu) : 0u;
             const RaStep r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1, raw2, false, false, life);
@@ -1731,6 +1744,7 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
     uint64_t life = 0x1Full;
     uint64_t valid = 0;  // lane 0 of a slot: region-0 entries holding a virtual register
     uint64_t mkd = 0;    // lane 0: entries whose register is known to be marked already
+    uint4 wn = make_uint4(0, 0, 0, 0);  // the word of the next step
     __syncthreads();
     for (uint32_t i = 0; i < wmax; ++i) {
         if (i % WALK_CHUNK == 0) {
@@ -1747,13 +1761,15 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
                 for (uint32_t r = 0; r < 16; ++r) words[kk][g0 + 2u * (16u * h + r)] = tmp[r];
             }
             __syncthreads();
+            wn = words[0][lane];
         }
         const bool on = (int32_t)i < (int32_t)size;
+        const uint4 w = wn;
+        if ((i + 1) % WALK_CHUNK != 0) wn = words[(i + 1) % WALK_CHUNK][lane];  // a step ahead
         RaStep r{};
         uint64_t look = 0;
         uint32_t raw1 = 0, raw2 = 0;
         if (on) {
-            const uint4 w = words[i % WALK_CHUNK][lane];
             raw1 = virt_ok(d, w.y) ? get(w.y) : 0u;
             raw2 = virt_ok(d, w.z) ? get(w.z) : 0u;
             // this step's commit may clear a set swapped flag (an operand's, or a
@@ -1768,12 +1784,10 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
             life = r.newlife;
         }
         // the group's lookups of this step, at every lane of the group
-        uint32_t lo = (uint32_t)look, hi = (uint32_t)(look >> 32);
-#pragma unroll
-        for (int s = 1; s < 8; s <<= 1) { lo |= (uint32_t)__shfl_xor((int)lo, s, 64); hi |= (uint32_t)__shfl_xor((int)hi, s, 64); }
+        const uint64_t glook = sm_or8_64(look);
         if (k == 0) {
             // marks on the step-start snapshot, before any commit of the step
-            uint64_t m = (((uint64_t)hi << 32) | lo) & valid & ~mkd;
+            uint64_t m = glook & valid & ~mkd;
             while (m) {
                 const uint32_t reg = (uint32_t)__builtin_ctzll(m);
                 m &= m - 1;
@@ -1848,15 +1862,6 @@ constexpr uint32_t SM_MIN_FUNCS = 9;       // programs of one group keep k_ra_gr
 
 struct SmWave { uint32_t np, state_bytes, prog[SM_PROGS], seg[SM_PROGS]; };
 
-// OR of v over the 8 lanes of each group (DPP: swap neighbours, swap pairs,
-// then mirror each half row of 8).
-DEV uint32_t sm_or8(uint32_t v) {
-    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
-    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
-    return v;
-}
-DEV uint64_t sm_or8_64(uint64_t v) { return (uint64_t)sm_or8((uint32_t)v) | ((uint64_t)sm_or8((uint32_t)(v >> 32)) << 32); }
 DEV uint32_t sm_rl(uint32_t v, uint32_t l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l); }
 DEV uint64_t sm_rl64(uint64_t v, uint32_t l) { return (uint64_t)sm_rl((uint32_t)v, l) | ((uint64_t)sm_rl((uint32_t)(v >> 32), l) << 32); }
 
