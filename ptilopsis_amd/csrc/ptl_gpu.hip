@@ -1507,12 +1507,66 @@ DEV void wc_ring_set(WcRing &g, uint32_t k, uint32_t v) {
 // overwritten without a look at what it held.  Invariant: a register held in
 // the ring is current there; one held in its LDS slot and not in the ring is
 // current there; a register leaving the ring overwrites its LDS slot.
-DEV uint32_t wt_lds_get(const Dev &d, uint32_t (*c)[64], uint32_t lane, uint32_t start, uint32_t x) {
-    const uint32_t e = c[x % WALK_CACHE][lane];
-    if (e && (e >> 10) == x) return e & 0x1FFu;
-    const uint32_t val = d.sym[start + x];
-    c[x % WALK_CACHE][lane] = (x << 10) | 0x200u | val;
-    return val;
+//
+// A step reads its two operands' states without branching on where they are:
+// both ring slots and both cache slots are read for every lane, and the lanes
+// whose operand is in neither (or is another function's register) load it from
+// HBM inside one wave-uniform branch that waits for its own loads.  raw1 / raw2
+// are 0 for an operand that is not a virtual register (one past the register
+// space is flagged, as virt_ok does).
+template <uint32_t WALK_RING>
+DEV void walk_gets(const Dev &d, uint32_t (*c)[64], const WcRing &ring, uint32_t lane, uint32_t start,
+                   uint32_t size, bool con, bool on, uint32_t y, uint32_t z, uint32_t &raw1, uint32_t &raw2) {
+    const bool n1 = on && (int32_t)y > 63, n2 = on && (int32_t)z > 63;
+    if ((n1 && y - 64u >= d.W0) || (n2 && z - 64u >= d.W0)) atomicOr(d.flags, EF_OOB);
+    const bool v1 = n1 && y - 64u < d.W0, v2 = n2 && z - 64u < d.W0;
+    const uint32_t x1 = y - 64u - start, x2 = z - 64u - start;
+    const bool l1 = v1 && con && x1 < size, l2 = v2 && con && x2 < size;
+    const uint32_t e1 = wc_ring_at(ring, x1 & WALK_RING), e2 = wc_ring_at(ring, x2 & WALK_RING);
+    const uint32_t c1 = c[x1 % WALK_CACHE][lane], c2 = c[x2 % WALK_CACHE][lane];
+    const bool r1 = e1 != 0u && (e1 >> 10) == x1, r2 = e2 != 0u && (e2 >> 10) == x2;
+    const bool m1 = v1 && !(l1 && (r1 || (c1 != 0u && (c1 >> 10) == x1)));
+    const bool m2 = v2 && !(l2 && (r2 || (c2 != 0u && (c2 >> 10) == x2)));
+    raw1 = v1 ? ((r1 ? e1 : c1) & 0x1FFu) : 0u;
+    raw2 = v2 ? ((r2 ? e2 : c2) & 0x1FFu) : 0u;
+    if (__ballot(m1 || m2)) {
+        // every lane loads (entry 0 where it hit), so that both loads issue
+        // back to back under one wait
+        const uint32_t g1 = d.sym[m1 ? y - 64u : 0u];
+        const uint32_t g2 = d.sym[m2 ? z - 64u : 0u];
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): no load is outstanding past this branch
+        if (m1) { raw1 = g1; if (l1) c[x1 % WALK_CACHE][lane] = (x1 << 10) | 0x200u | g1; }
+        if (m2) { raw2 = g2; if (l2) c[x2 % WALK_CACHE][lane] = (x2 << 10) | 0x200u | g2; }
+    }
+}
+
+// Stages the words of steps i .. i+WALK_CHUNK-1 of the 64 walked functions in
+// LDS: thread t loads word i + t % 32 of functions t / 32, t / 32 + 2, ... (two
+// batches of 16 loads, consecutive threads on consecutive words of one
+// function).  Each load's function start and size come from the lanes that hold
+// them (readlane), and past a function's end the load reads word 0 and stages
+// zeros, so that the loads issue back to back.
+DEV void walk_stage(const Dev &d, uint4 (*words)[64], uint32_t i, uint32_t lane, uint32_t start, uint32_t size) {
+    const uint32_t kk = lane % WALK_CHUNK;
+    const bool odd = lane >= WALK_CHUNK;
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        uint4 tmp[16];
+#pragma unroll
+        for (uint32_t r = 0; r < 16; ++r) {
+            const uint32_t g = 2u * (16u * h + r);
+            const uint32_t st0 = (uint32_t)__builtin_amdgcn_readlane((int)start, (int)g);
+            const uint32_t st1 = (uint32_t)__builtin_amdgcn_readlane((int)start, (int)g + 1);
+            const uint32_t sz0 = (uint32_t)__builtin_amdgcn_readlane((int)size, (int)g);
+            const uint32_t sz1 = (uint32_t)__builtin_amdgcn_readlane((int)size, (int)g + 1);
+            const uint32_t st = odd ? st1 : st0, sz = odd ? sz1 : sz0;
+            const bool in = i + kk < sz;
+            const uint4 wv = d.w0_f4[in ? st + i + kk : 0u];
+            tmp[r] = in ? wv : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < 16; ++r) words[kk][(odd ? 1u : 0u) + 2u * (16u * h + r)] = tmp[r];
+    }
 }
 
 
@@ -1523,7 +1577,6 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
     __shared__ uint32_t smin[64], smax[64];
     __shared__ uint32_t cache[WALK_CACHE][64];
     __shared__ uint4 words[WALK_CHUNK][64];
-    __shared__ uint32_t fstart[64], fsize[64];
     const uint32_t lane = threadIdx.x;
     const uint32_t t = blockIdx.x * 64u + lane;
     const uint32_t f = t < nw ? permw[t] : d.F;
@@ -1563,14 +1616,7 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
     const bool publishes = done && lead && cbase != RA0_NOPUB;
     const bool con = size < (1u << 22);  // cache on (tags hold 22 bits of word offset)
     WcRing ring{0, 0, 0, 0, 0, 0, 0, 0};
-    // state of virtual register v (inside [64, 64 + W0)); then its update
-    auto get = [&](uint32_t v) -> uint32_t {
-        const uint32_t x = v - 64u - start;
-        if (!con || x >= size) return d.sym[v - 64u];
-        const uint32_t e = wc_ring_at(ring, x & WALK_RING);
-        if (e && (e >> 10) == x) return e & 0x1FFu;
-        return wt_lds_get(d, cache, lane, start, x);
-    };
+    // update of virtual register v (inside [64, 64 + W0))
     auto put = [&](uint32_t v, uint32_t val, bool own) {  // own: v is defined by the word being committed
         const uint32_t x = v - 64u - start;
         d.sym[v - 64u] = (uint16_t)val;  // write-through
@@ -1582,7 +1628,6 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
         wc_ring_set(ring, x & WALK_RING, (x << 10) | 0x200u | val);
     };
     for (uint32_t k = 0; k < WALK_CACHE; ++k) cache[k][lane] = 0;
-    fstart[lane] = start; fsize[lane] = size;
     smin[lane] = 0xFFFFFFFFu; smax[lane] = 0;
     uint32_t wmax = size;
     for (int s = 32; s > 0; s >>= 1) { uint32_t o = __shfl_xor(wmax, s, 64); wmax = wmax > o ? wmax : o; }
@@ -1591,27 +1636,17 @@ __global__ __launch_bounds__(64) void k_ra_walk(Dev d, uint32_t *gmin_all, uint3
     __syncthreads();
     for (uint32_t i = 0; i < wmax; ++i) {
         if (i % WALK_CHUNK == 0) {
-            // thread t stages word i + t % 32 of functions t / 32, t / 32 + 2, ...: two batches of 16 loads
-            const uint32_t k = lane % WALK_CHUNK, g0 = lane / WALK_CHUNK;
-#pragma unroll
-            for (uint32_t h = 0; h < 2; ++h) {
-                uint4 tmp[16];
-#pragma unroll
-                for (uint32_t r = 0; r < 16; ++r) {
-                    const uint32_t g = g0 + 2u * (16u * h + r);
-                    tmp[r] = i + k < fsize[g] ? d.w0_f4[fstart[g] + i + k] : make_uint4(0, 0, 0, 0);
-                }
-#pragma unroll
-                for (uint32_t r = 0; r < 16; ++r) words[k][g0 + 2u * (16u * h + r)] = tmp[r];
-            }
+            walk_stage(d, words, i, lane, start, size);
             __syncthreads();
             wn = words[0][lane];
         }
         const uint4 w = wn;
         if ((i + 1) % WALK_CHUNK != 0) wn = words[(i + 1) % WALK_CHUNK][lane];  // a step ahead
-        if ((int32_t)i < (int32_t)size) {
-            const uint32_t raw1 = virt_ok(d, w.y) ? (get(w.y) & 0xFFu) : 0u;
-            const uint32_t raw2 = virt_ok(d, w.z) ? (get(w.z) & 0xFFu) : 0u;
+        const bool on = (int32_t)i < (int32_t)size;
+        uint32_t raw1, raw2;
+        walk_gets<WALK_RING>(d, cache, ring, lane, start, size, con, on, w.y, w.z, raw1, raw2);
+        if (on) {
+            raw1 &= 0xFFu; raw2 &= 0xFFu;
             const RaStep r = ra_eval_raw(w.w, w.x, w.y, w.z, raw1, raw2, false, false, life);
             xref_check_fields(d, f, start, size, r.ord, r.rs1, r.rs2);
             if (one) {
@@ -1694,7 +1729,6 @@ template <uint32_t WALK_RING>
 __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, uint32_t np) {
     __shared__ uint32_t cache[WALK_CACHE][64];
     __shared__ uint4 words[WALK_CHUNK][64];
-    __shared__ uint32_t fstart[64], fsize[64];
     __shared__ int32_t Sv[8][64];
     const uint32_t lane = threadIdx.x, slot = lane >> 3, k = lane & 7u;
     const uint32_t pi = blockIdx.x * 8u + slot;
@@ -1708,13 +1742,6 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
     const uint32_t size = act ? d.fsizes[f] : 0u;
     const bool con = size < (1u << 22);
     WcRing ring{0, 0, 0, 0, 0, 0, 0, 0};
-    auto get = [&](uint32_t v) -> uint32_t {
-        const uint32_t x = v - 64u - start;
-        if (!con || x >= size) return d.sym[v - 64u];
-        const uint32_t e = wc_ring_at(ring, x & WALK_RING);
-        if (e && (e >> 10) == x) return e & 0x1FFu;
-        return wt_lds_get(d, cache, lane, start, x);
-    };
     auto put = [&](uint32_t v, uint32_t val, bool own) {  // own: v is defined by the word being committed
         const uint32_t x = v - 64u - start;
         d.sym[v - 64u] = (uint16_t)val;  // write-through
@@ -1726,7 +1753,6 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
         wc_ring_set(ring, x & WALK_RING, (x << 10) | 0x200u | val);
     };
     for (uint32_t q = 0; q < WALK_CACHE; ++q) cache[q][lane] = 0;
-    fstart[lane] = start; fsize[lane] = size;
     Sv[slot][k] = -1; Sv[slot][k + 8] = -1; Sv[slot][k + 16] = -1; Sv[slot][k + 24] = -1;
     Sv[slot][k + 32] = -1; Sv[slot][k + 40] = -1; Sv[slot][k + 48] = -1; Sv[slot][k + 56] = -1;
     uint32_t wmax = size;
@@ -1748,18 +1774,7 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
     __syncthreads();
     for (uint32_t i = 0; i < wmax; ++i) {
         if (i % WALK_CHUNK == 0) {
-            const uint32_t kk = lane % WALK_CHUNK, g0 = lane / WALK_CHUNK;
-#pragma unroll
-            for (uint32_t h = 0; h < 2; ++h) {
-                uint4 tmp[16];
-#pragma unroll
-                for (uint32_t r = 0; r < 16; ++r) {
-                    const uint32_t g = g0 + 2u * (16u * h + r);
-                    tmp[r] = i + kk < fsize[g] ? d.w0_f4[fstart[g] + i + kk] : make_uint4(0, 0, 0, 0);
-                }
-#pragma unroll
-                for (uint32_t r = 0; r < 16; ++r) words[kk][g0 + 2u * (16u * h + r)] = tmp[r];
-            }
+            walk_stage(d, words, i, lane, start, size);
             __syncthreads();
             wn = words[0][lane];
         }
@@ -1768,10 +1783,9 @@ __global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, u
         if ((i + 1) % WALK_CHUNK != 0) wn = words[(i + 1) % WALK_CHUNK][lane];  // a step ahead
         RaStep r{};
         uint64_t look = 0;
-        uint32_t raw1 = 0, raw2 = 0;
+        uint32_t raw1, raw2;
+        walk_gets<WALK_RING>(d, cache, ring, lane, start, size, con, on, w.y, w.z, raw1, raw2);
         if (on) {
-            raw1 = virt_ok(d, w.y) ? get(w.y) : 0u;
-            raw2 = virt_ok(d, w.z) ? get(w.z) : 0u;
             // this step's commit may clear a set swapped flag (an operand's, or a
             // destination that is not the word's own new register): forget what
             // is marked
