@@ -2228,8 +2228,9 @@ struct Ra0Fn {
     uint32_t j, start, size;              // group, first word, word count
     uint64_t life;
     uint64_t own;                         // snapshot registers this function may own
-    uint32_t ord, rs1, rs2;               // step i (reloaded from LDS in every phase)
+    uint32_t ord, rs1, rs2;               // step i (kept, or reloaded from LDS in every phase)
     uint4 nw;                             // step i+1: rd, rs1, rs2, instruction (zeros past the end)
+    uint32_t rr;                          // step i's result: r1r | r2r << 8 | rdr << 16
     uint32_t pk;                          // step i: raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18 | OP-FP << 31
 };
 
@@ -2290,8 +2291,11 @@ DEV uint32_t ra0_flips(uint64_t o, const uint32_t *flip) {
 
 constexpr uint32_t RA0_LDS_FNS = 8192;  // lane-0 functions whose current fields fit in LDS
 
+// A function's fields of step i, from LDS where its thread does not keep them
+// in registers across the step (KEEP: at most three functions per thread).
+template <bool KEEP>
 DEV void ra0_cur(Ra0Fn &f, const uint32_t *crd, const uint32_t *crs1, const uint32_t *crs2, uint32_t j) {
-    f.ord = crd[j]; f.rs1 = crs1[j]; f.rs2 = crs2[j];
+    if (!KEEP) { f.ord = crd[j]; f.rs1 = crs1[j]; f.rs2 = crs2[j]; }
 }
 
 DEV void ra0_lookups(uint32_t *mn, uint32_t *mx, const RaStep &r, uint32_t j) {
@@ -2360,7 +2364,6 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                                                    const uint32_t *done, const uint32_t *need, uint32_t copies,
                                                    uint64_t stride) {
     __shared__ uint32_t cur_rd[NL], cur_rs1[NL], cur_rs2[NL];
-    __shared__ uint32_t rrs[NL];                              // step result per group: r1r | r2r << 8 | rdr << 16
     __shared__ int32_t Sv[2][64], So[2][64];                  // region-0 snapshot of step i / i+1
     // lookups by all groups but the owners, and by the walk: the walk's row of
     // a step is folded in two steps ahead, into the table cleared after its last
@@ -2375,6 +2378,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
     __shared__ int changed;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     const uint32_t wid = tid >> 6, nwv = nth >> 6, lane = tid & 63u;
+    constexpr bool KEEP = PER <= 3;  // a function's fields of step i stay in registers
     // with one table copy per loading wave, a loading wave holds its copy's
     // row of step i+2 in registers, loaded a step before it is staged
     const uint32_t k0 = nwv - 1u - wid;
@@ -2394,7 +2398,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
     for (int q = 0; q < PER; ++q) {
         const uint32_t t = tid + q * nth;
         Ra0Fn &f = fn[q];
-        f.j = 0; f.start = 0; f.size = 0; f.life = 0x1Full; f.own = 0;
+        f.j = 0; f.start = 0; f.size = 0; f.life = 0x1Full; f.own = 0; f.rr = 0;
         if (t < L) {
             f.j = perm[t];
             if (f.j < L) { f.start = d.fstarts[fbase + 8u * f.j]; f.size = d.fsizes[fbase + 8u * f.j]; }
@@ -2444,15 +2448,15 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             Ra0Fn &f = fn[q];
             const uint32_t j = f.j;
             if ((int32_t)i < (int32_t)f.size) {
-                ra0_cur(f, cur_rd, cur_rs1, cur_rs2, j);
+                ra0_cur<KEEP>(f, cur_rd, cur_rs1, cur_rs2, j);
                 xref_check_fields(d, fbase + 8u * j, f.start, f.size, f.ord, f.rs1, f.rs2);
                 const RaStep r = ra0_step(f);
                 if (f.own) {
                     ra0_lookups(Omin[c], Omax[c], r, j);
-                    rrs[j] = r.r1r | (r.r2r << 8) | (r.rdr << 16);  // final unless a settle round re-evaluates
+                    f.rr = r.r1r | (r.r2r << 8) | (r.rdr << 16);  // final unless a settle round re-evaluates
                 } else {
                     ra0_lookups(Lmin[c], Lmax[c], r, j);
-                    rrs[j] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
+                    f.rr = r.r1r | (r.r2r << 8) | (r.rdr << 16);
                     f.life = r.newlife;
                     ra0_keys(wkey[c], r, j);
                 }
@@ -2491,10 +2495,10 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                     if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
                         fn[q].pk = (fn[q].pk & 0x8003FFFFu) |
                                    (ra0_flips(ra0_owned(fn[q].own & fm, Sv[c], So[c], j), flip) << 18);
-                        ra0_cur(fn[q], cur_rd, cur_rs1, cur_rs2, j);
+                        ra0_cur<KEEP>(fn[q], cur_rd, cur_rs1, cur_rs2, j);
                         const RaStep r = ra0_step(fn[q]);
                         ra0_lookups(Omin[c], Omax[c], r, j);
-                        rrs[j] = r.r1r | (r.r2r << 8) | (r.rdr << 16);
+                        fn[q].rr = r.r1r | (r.r2r << 8) | (r.rdr << 16);
                     }
                 }
                 lds_barrier();
@@ -2525,9 +2529,9 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         for (int q = 0; q < PER; ++q) {
             const uint32_t j = fn[q].j;
             if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
-                const uint32_t rr = rrs[j];
+                const uint32_t rr = fn[q].rr;
                 RaStep r;
-                r.ord = cur_rd[j];
+                r.ord = KEEP ? fn[q].ord : cur_rd[j];
                 r.r1r = rr & 0xFFu; r.r2r = (rr >> 8) & 0xFFu; r.rdr = (rr >> 16) & 0xFFu;
                 const uint64_t clr = ((1ull << r.r1r) | (1ull << r.r2r)) & ~1ull;
                 fn[q].life = (fn[q].life & ~clr) | (1ull << r.rdr);
@@ -2571,8 +2575,8 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             if (t >= L) continue;
             uint64_t won = 0;
             if ((int32_t)i < (int32_t)f.size) {
-                ra0_cur(f, cur_rd, cur_rs1, cur_rs2, j);
-                const uint32_t rr = rrs[j];
+                ra0_cur<KEEP>(f, cur_rd, cur_rs1, cur_rs2, j);
+                const uint32_t rr = f.rr;
                 RaStep r;
                 r.ord = f.ord; r.rs1 = f.rs1; r.rs2 = f.rs2;
                 r.r1r = rr & 0xFFu; r.r2r = (rr >> 8) & 0xFFu; r.rdr = (rr >> 16) & 0xFFu;
@@ -2585,6 +2589,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 ra0_commit(d, r, (f.pk & 0xFFu) | (((f.pk >> 16) & 1u) << 8), ((f.pk >> 8) & 0xFFu) | (((f.pk >> 17) & 1u) << 8));
                 const bool nx = (int32_t)(i + 1) < (int32_t)f.size;
                 cur_rd[j] = nx ? f.nw.x : 0u; cur_rs1[j] = nx ? f.nw.y : 0u; cur_rs2[j] = nx ? f.nw.z : 0u;
+                if (KEEP) { f.ord = nx ? f.nw.x : 0u; f.rs1 = nx ? f.nw.y : 0u; f.rs2 = nx ? f.nw.z : 0u; }
             }
             if (f.own) {
                 const uint64_t o = (f.own & ~kprev) | ra0_owned(f.own & kprev, Sv[c], So[c], j);
