@@ -2231,7 +2231,9 @@ struct Ra0Fn {
     uint32_t ord, rs1, rs2;               // step i (kept, or reloaded from LDS in every phase)
     uint4 nw;                             // step i+1: rd, rs1, rs2, instruction (zeros past the end)
     uint32_t rr;                          // step i's result: r1r | r2r << 8 | rdr << 16
-    uint32_t pk;                          // step i: raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18 | OP-FP << 31
+    uint32_t pk;                          // step i: raw1 | raw2 << 8 | s1 << 16 | s2 << 17 | flips << 18 |
+                                          // won << 20 | OP-FP << 31 (won: 0x40 | the snapshot entry its
+                                          // step i-1 commit won, 0 for none)
 };
 
 DEV RaStep ra0_step(const Ra0Fn &f) {
@@ -2493,7 +2495,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 for (int q = 0; q < PER; ++q) {
                     const uint32_t j = fn[q].j;
                     if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
-                        fn[q].pk = (fn[q].pk & 0x8003FFFFu) |
+                        fn[q].pk = (fn[q].pk & ~(3u << 18)) |
                                    (ra0_flips(ra0_owned(fn[q].own & fm, Sv[c], So[c], j), flip) << 18);
                         ra0_cur<KEEP>(fn[q], cur_rd, cur_rs1, cur_rs2, j);
                         const RaStep r = ra0_step(fn[q]);
@@ -2567,11 +2569,13 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             const uint64_t kb = __ballot(wkey[c][tid] != 0);
             if (tid == 0) { keyed[c][0] = (uint32_t)kb; keyed[c][1] = (uint32_t)(kb >> 32); }
         }
+        uint32_t wb[PER];  // the entry each function's commit wins: 0x40 | register, 0 for none
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const uint32_t t = tid + q * nth;
             Ra0Fn &f = fn[q];
             const uint32_t j = f.j;
+            wb[q] = 0;
             if (t >= L) continue;
             uint64_t won = 0;
             if ((int32_t)i < (int32_t)f.size) {
@@ -2582,7 +2586,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 r.r1r = rr & 0xFFu; r.r2r = (rr >> 8) & 0xFFu; r.rdr = (rr >> 16) & 0xFFu;
                 if (r.ord && wkey[c][r.rdr] == (((j + 1u) << 2) | 1u)) {
                     Sv[n][r.rdr] = (int32_t)r.ord; So[n][r.rdr] = (int32_t)j;
-                    if ((int32_t)r.ord >= 64) won = 1ull << r.rdr;
+                    if ((int32_t)r.ord >= 64) { won = 1ull << r.rdr; wb[q] = 0x40u | r.rdr; }
                 }
                 if (r.r1r != r.rdr && wkey[c][r.r1r] == (((j + 1u) << 2) | 2u)) { Sv[n][r.r1r] = -1; So[n][r.r1r] = -1; }
                 if (r.r2r != r.rdr && wkey[c][r.r2r] == (((j + 1u) << 2) | 3u)) { Sv[n][r.r2r] = -1; So[n][r.r2r] = -1; }
@@ -2592,7 +2596,12 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 if (KEEP) { f.ord = nx ? f.nw.x : 0u; f.rs1 = nx ? f.nw.y : 0u; f.rs2 = nx ? f.nw.z : 0u; }
             }
             if (f.own) {
-                const uint64_t o = (f.own & ~kprev) | ra0_owned(f.own & kprev, Sv[c], So[c], j);
+                // the entries it owns now: those it owned that the commit of step
+                // i-1 left alone, and the one it won in that commit (a won entry
+                // is keyed, and a keyed entry is won by one group alone)
+                const uint32_t pw = (f.pk >> 20) & 0x7Fu;
+                const uint64_t wprev = (pw & 0x40u) ? 1ull << (pw & 63u) : 0ull;
+                const uint64_t o = (f.own & ~kprev) | (f.own & kprev & wprev);
                 uint64_t m = o & marked;
                 while (m) {
                     const uint32_t reg = (uint32_t)__builtin_ctzll(m);
@@ -2626,7 +2635,7 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         // on any path into the next step)
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            fn[q].pk = ra0_pack(g1[q], g2[q]);
+            fn[q].pk = ra0_pack(g1[q], g2[q]) | (wb[q] << 20);
             asm volatile("" : "+v"(fn[q].pk) : : "memory");  // packed here, before the loads below
         }
         // the walk's row of step i+3, staged at the end of step i+1, and the
