@@ -2549,7 +2549,8 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             if (v >= 64 && o >= 0 && mx != 0) {
                 mark = true;
                 const uint32_t uv = (uint32_t)v;
-                if (cur_rd[o] == uv || cur_rs1[o] == uv || cur_rs2[o] == uv) mark = mx > (uint32_t)o + 1u;
+                const uint32_t a = cur_rd[o], b = cur_rs1[o], e = cur_rs2[o];  // read together
+                if ((a == uv) | (b == uv) | (e == uv)) mark = mx > (uint32_t)o + 1u;
             }
             const uint64_t m = __ballot(mark);
             if (tid == 0) { mk[c][0] = (uint32_t)m; mk[c][1] = (uint32_t)(m >> 32); }
@@ -2584,12 +2585,17 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 RaStep r;
                 r.ord = f.ord; r.rs1 = f.rs1; r.rs2 = f.rs2;
                 r.r1r = rr & 0xFFu; r.r2r = (rr >> 8) & 0xFFu; r.rdr = (rr >> 16) & 0xFFu;
-                if (r.ord && wkey[c][r.rdr] == (((j + 1u) << 2) | 1u)) {
+                // the three keys read together (registers are below 64), then compared
+                const uint32_t kd = wkey[c][r.rdr & 63u], k1 = wkey[c][r.r1r & 63u], k2 = wkey[c][r.r2r & 63u];
+                const bool wd = (r.ord != 0u) & (kd == (((j + 1u) << 2) | 1u));
+                const bool w1 = (r.r1r != r.rdr) & (k1 == (((j + 1u) << 2) | 2u));
+                const bool w2 = (r.r2r != r.rdr) & (k2 == (((j + 1u) << 2) | 3u));
+                if (wd) {
                     Sv[n][r.rdr] = (int32_t)r.ord; So[n][r.rdr] = (int32_t)j;
                     if ((int32_t)r.ord >= 64) { won = 1ull << r.rdr; wb[q] = 0x40u | r.rdr; }
                 }
-                if (r.r1r != r.rdr && wkey[c][r.r1r] == (((j + 1u) << 2) | 2u)) { Sv[n][r.r1r] = -1; So[n][r.r1r] = -1; }
-                if (r.r2r != r.rdr && wkey[c][r.r2r] == (((j + 1u) << 2) | 3u)) { Sv[n][r.r2r] = -1; So[n][r.r2r] = -1; }
+                if (w1) { Sv[n][r.r1r] = -1; So[n][r.r1r] = -1; }
+                if (w2) { Sv[n][r.r2r] = -1; So[n][r.r2r] = -1; }
                 ra0_commit(d, r, (f.pk & 0xFFu) | (((f.pk >> 16) & 1u) << 8), ((f.pk >> 8) & 0xFFu) | (((f.pk >> 17) & 1u) << 8));
                 const bool nx = (int32_t)(i + 1) < (int32_t)f.size;
                 cur_rd[j] = nx ? f.nw.x : 0u; cur_rs1[j] = nx ? f.nw.y : 0u; cur_rs2[j] = nx ? f.nw.z : 0u;
