@@ -2274,20 +2274,17 @@ DEV void ra0_commit(const Dev &d, const RaStep &r, uint32_t old1, uint32_t old2)
     if (v2 && r.r2r != old2) d.sym[r.rs2 - 64u] = (uint16_t)r.r2r;
 }
 
-// The snapshot registers of `own` that group j owns in snapshot (Sv, So).
-DEV uint64_t ra0_owned(uint64_t own, const int32_t *Sv, const int32_t *So, uint32_t j) {
-    uint64_t o = 0;
+// The flip bits of the snapshot entries of `own` that group j owns, each
+// entry's owner, register and flip bits read together.
+DEV uint32_t ra0_owned_flips(uint64_t own, const int32_t *Sv, const int32_t *So, const uint32_t *flip, uint32_t j) {
+    uint32_t f = 0;
     while (own) {
         const uint32_t r = (uint32_t)__builtin_ctzll(own);
         own &= own - 1;
-        if (So[r] == (int32_t)j && Sv[r] >= 64) o |= 1ull << r;
+        const int32_t so = So[r], sv = Sv[r];
+        const uint32_t fl = flip[r];
+        f |= fl & (0u - (uint32_t)((so == (int32_t)j) & (sv >= 64)));
     }
-    return o;
-}
-
-DEV uint32_t ra0_flips(uint64_t o, const uint32_t *flip) {
-    uint32_t f = 0;
-    while (o) { const uint32_t r = (uint32_t)__builtin_ctzll(o); o &= o - 1; f |= flip[r]; }
     return f;
 }
 
@@ -2472,7 +2469,8 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             Omin[c][tid] = 0xFFFFFFFFu; Omax[c][tid] = 0;  // cleared for the settle rounds and step i+2
             uint32_t nb = 0;
             const int32_t o = So[c][tid], v = Sv[c][tid];
-            if (v >= 64 && o >= 0 && min(Lmin[c][tid], om) < (uint32_t)o) {
+            const uint32_t lm = Lmin[c][tid];  // read with the entry, not behind its test
+            if ((v >= 64) & (o >= 0) & (min(lm, om) < (uint32_t)o)) {
                 if (cur_rs1[o] == (uint32_t)v) nb |= 1u;
                 if (cur_rs2[o] == (uint32_t)v) nb |= 2u;
             }
@@ -2484,19 +2482,21 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         if (tid >= cb)
             for (uint32_t k = tid - cb; k < 64; k += nth - cb) wkey[n][k] = 0;
         lds_barrier();
+        // (the flipped entries read with the flag, for the first round)
+        const uint64_t fm0 = ((uint64_t)fmk[1] << 32) | fmk[0];
         if (changed) {
             // settle intra-step flips among the owners: re-evaluate until the
             // flip bits are stable (lower groups settle first, so this reaches
             // the in-order result); O[c] was cleared when wave 0 last read it
             for (uint32_t round = 0;; ++round) {
                 // only entries with flip bits can give an owner flips
-                const uint64_t fm = ((uint64_t)fmk[1] << 32) | fmk[0];
+                const uint64_t fm = round == 0 ? fm0 : ((uint64_t)fmk[1] << 32) | fmk[0];
 #pragma unroll
                 for (int q = 0; q < PER; ++q) {
                     const uint32_t j = fn[q].j;
                     if (fn[q].own && (int32_t)i < (int32_t)fn[q].size) {
                         fn[q].pk = (fn[q].pk & ~(3u << 18)) |
-                                   (ra0_flips(ra0_owned(fn[q].own & fm, Sv[c], So[c], j), flip) << 18);
+                                   (ra0_owned_flips(fn[q].own & fm, Sv[c], So[c], flip, j) << 18);
                         ra0_cur<KEEP>(fn[q], cur_rd, cur_rs1, cur_rs2, j);
                         const RaStep r = ra0_step(fn[q]);
                         ra0_lookups(Omin[c], Omax[c], r, j);
@@ -2509,7 +2509,8 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                     Omin[c][tid] = 0xFFFFFFFFu; Omax[c][tid] = 0;
                     uint32_t nb = 0;
                     const int32_t o = So[c][tid], v = Sv[c][tid];
-                    if (v >= 64 && o >= 0 && min(Lmin[c][tid], om) < (uint32_t)o) {
+                    const uint32_t lm = Lmin[c][tid];
+                    if ((v >= 64) & (o >= 0) & (min(lm, om) < (uint32_t)o)) {
                         if (cur_rs1[o] == (uint32_t)v) nb |= 1u;
                         if (cur_rs2[o] == (uint32_t)v) nb |= 2u;
                     }
