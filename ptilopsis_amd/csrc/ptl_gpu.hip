@@ -229,6 +229,92 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(Op op, uint64_t n, con
     }
 }
 
+// ------------------------------------------------------- single-pass scan
+// The same exclusive scan in one launch with decoupled look-back: each element
+// is produced once.  A workgroup takes the next tile from a ticket counter, so
+// tiles start in order and every tile it waits on is already running; it
+// scans its tile as above, publishes the tile's aggregate, and wave 0 looks
+// back over the published words of its predecessors 64 at a time, summing
+// aggregates back to the nearest inclusive prefix (tile -1 counts as an
+// inclusive 0) and waiting while a nearer word is unpublished.  It then
+// publishes its own inclusive prefix and the tile emits.  Status word of a
+// tile: value | flag << 32 (1 aggregate, 2 inclusive), all zero before the
+// launch; the ticket counter is the word after the last tile's.  A status
+// word carries its own value, so no other data is ordered by it: the stores
+// and polls are relaxed device-scope atomics.
+constexpr uint64_t SCAN_AGG = 1ull << 32, SCAN_INCL = 2ull << 32;
+
+template <class Op>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_dlb(Op op, uint64_t n, unsigned long long *status,
+                                                         uint32_t ntiles, uint32_t *total) {
+    __shared__ uint32_t s_w[SCAN_BLOCK / 64];
+    __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 16];
+    __shared__ uint32_t s_tile, s_pre;
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_tile = atomicAdd(reinterpret_cast<uint32_t *>(status + ntiles), 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t tb = (uint64_t)tile * SCAN_TILE;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const uint32_t e = (uint32_t)k * SCAN_BLOCK + threadIdx.x;
+        s_v[e + e / 16u] = tb + e < n ? op.value(tb + e) : 0u;  // element e at e + e/16 (one pad word per 16)
+    }
+    __syncthreads();
+    uint32_t v[SCAN_ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = s_v[threadIdx.x * (SCAN_ITEMS + 1) + k];  // element t*16 + k
+        sum += v[k];
+    }
+    const uint32_t x = wave_incl_scan(sum, lane);
+    if (lane == 63) s_w[wid] = x;
+    __syncthreads();
+    uint32_t agg = 0, run = x - sum;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+        agg += s_w[w];
+        if ((uint32_t)w < wid) run += s_w[w];
+    }
+    if (wid == 0) {
+        if (lane == 0)
+            __hip_atomic_store(status + tile, (unsigned long long)agg | (tile == 0 ? SCAN_INCL : SCAN_AGG),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t pre = 0;
+        int64_t j = (int64_t)tile - 1;  // nearest predecessor not yet summed (lane 0 reads it)
+        while (j >= 0) {
+            const int64_t idx = j - (int64_t)lane;
+            const unsigned long long st =
+                idx >= 0 ? __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : SCAN_INCL;
+            const uint64_t incl = __ballot((st >> 32) == 2u), none = __ballot((st >> 32) == 0u);
+            const uint32_t stop = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+            if (none && (uint32_t)__builtin_ctzll(none) < stop) {  // a nearer tile has not published yet
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            pre += wave_sum(lane <= stop ? (uint32_t)st : 0u);
+            if (incl) break;
+            j -= 64;
+        }
+        if (lane == 0) {
+            if (tile > 0)
+                __hip_atomic_store(status + tile, (unsigned long long)(pre + agg) | SCAN_INCL, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            s_pre = pre;
+            if (total && tile == ntiles - 1) *total = pre + agg;
+        }
+    }
+    __syncthreads();
+    run += s_pre;
+    const uint64_t base = tb + (uint64_t)threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const uint64_t i = base + k;
+        if (i < n) op.emit(i, run, v[k]);
+        run += v[k];
+    }
+}
+
 // ----------------------------------------------------------- device state
 struct Dev {
     // inputs: P programs concatenated node-wise, parents global
