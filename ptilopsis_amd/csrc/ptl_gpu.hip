@@ -2658,12 +2658,17 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
             if (tid == 0) { keyed[c][0] = (uint32_t)kb; keyed[c][1] = (uint32_t)(kb >> 32); }
         }
         uint32_t wb[PER];  // the entry each function's commit wins: 0x40 | register, 0 for none
+        // the operand states of step i+1 (sym entries), loaded right after each
+        // function's commit (the commit's stores come first in program order);
+        // a mark of one of them that follows, and the OP-FP bit, are added at
+        // packing (pm: bit 0 / 1 operand marked, bit 2 OP-FP)
+        uint32_t g1[PER], g2[PER], pm[PER];
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const uint32_t t = tid + q * nth;
             Ra0Fn &f = fn[q];
             const uint32_t j = f.j;
-            wb[q] = 0;
+            wb[q] = 0; g1[q] = 0; g2[q] = 0; pm[q] = 0;
             if (t >= L) continue;
             uint64_t won = 0;
             if ((int32_t)i < (int32_t)f.size) {
@@ -2688,6 +2693,10 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                 cur_rd[j] = nx ? f.nw.x : 0u; cur_rs1[j] = nx ? f.nw.y : 0u; cur_rs2[j] = nx ? f.nw.z : 0u;
                 if (KEEP) { f.ord = nx ? f.nw.x : 0u; f.rs1 = nx ? f.nw.y : 0u; f.rs2 = nx ? f.nw.z : 0u; }
             }
+            if ((int32_t)(i + 1) < (int32_t)f.size) {
+                ra0_gather(d, f.nw.y, f.nw.z, g1[q], g2[q]);
+                pm[q] = ra0_fpbit(f.nw.w) << 2;  // (not ORed into g1 here: that would wait for the load)
+            }
             if (f.own) {
                 // the entries it owns now: those it owned that the commit of step
                 // i-1 left alone, and the one it won in that commit (a won entry
@@ -2702,22 +2711,11 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
                     const uint32_t v = (uint32_t)Sv[c][reg];
                     if (v - 64u < d.W0) sym_mark(d, v - 64u);
                     else atomicOr(d.flags, EF_OOB);
+                    pm[q] |= (v == f.nw.y ? 1u : 0u) | (v == f.nw.z ? 2u : 0u);
                 }
                 f.own = o;
             }
             f.own |= won;
-        }
-        // loads for step i+1: its operands' state (after this step's stores, in
-        // program order) and the fields of step i+2
-        uint32_t g1[PER], g2[PER];
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            Ra0Fn &f = fn[q];
-            g1[q] = g2[q] = 0;
-            if ((int32_t)(i + 1) < (int32_t)f.size) {
-                ra0_gather(d, f.nw.y, f.nw.z, g1[q], g2[q]);
-                g1[q] |= ra0_fpbit(f.nw.w) << 31;
-            }
         }
         if (reg_rows) {
             if (k0 < copies && i + 2 < maxsize) { atomicMin(&Lmin[c][lane], rmn); atomicMax(&Lmax[c][lane], rmx); }
@@ -2728,7 +2726,8 @@ __global__ __launch_bounds__(1024) void k_ra_lane0(Dev d, const Ra0Prog *progs, 
         // on any path into the next step)
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            fn[q].pk = ra0_pack(g1[q], g2[q]) | (wb[q] << 20);
+            fn[q].pk = ra0_pack(g1[q] | ((pm[q] & 1u) << 8) | ((pm[q] >> 2) << 31), g2[q] | ((pm[q] & 2u) << 7)) |
+                       (wb[q] << 20);
             asm volatile("" : "+v"(fn[q].pk) : : "memory");  // packed here, before the loads below
         }
         // the walk's row of step i+3, staged at the end of step i+1, and the
