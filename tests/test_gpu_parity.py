@@ -41,7 +41,9 @@ def test_gpu_scan_kat(gpu_ctx):
     assert ex[0] == 0 and np.array_equal(ex[1:].astype(np.uint64), ref[:-1])
 
 
-@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, 1 << 22])
+# tile edges of the single-pass scan (4096 elements a tile) and of its
+# look-back window (64 tiles a poll)
+@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, 4095, 4096, 4097, 64 * 4096 + 1, 129 * 4096 - 1, 1 << 22])
 def test_gpu_scan_sizes(gpu_ctx, n):
     x = np.random.default_rng(n).integers(0, 1000, n).astype(np.uint32)
     ex = gpu_ctx.exclusive_scan(x)
