@@ -45,6 +45,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <future>
 #include <mutex>
 #include <type_traits>
 
