@@ -338,6 +338,7 @@ struct Dev {
     // isn_gen
     uint32_t *regs;          // 3n
     uint32_t *rank;          // n (sorted position of each node)
+    uint32_t *rinv;          // n (node at each sorted position), with rank
     uint64_t *owner;         // W0
     uint32_t W0;
     uint32_t *w0_instr, *w0_rd, *w0_rs1, *w0_rs2, *w0_jt;
@@ -695,29 +696,34 @@ __global__ __launch_bounds__(256) void k_gen_ties(Dev d) {
     }
 }
 
+// The words of isn_gen, one thread per word slot (W0 + 1 of them, the last a
+// zero pad): the slot's owner key names its writer, a node (through rinv when the
+// keys hold sorted positions) and one of its lanes, whose word is computed
+// again; a slot nobody writes is zero.  The five arrays are written in slot
+// order, each slot once, so they need no clearing beforehand.
 __global__ __launch_bounds__(256) void k_gen_write(Dev d, int use_rank) {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= d.n) return;
-    const GenNode N = gen_node(d, (uint32_t)i);
-    const uint64_t k0 = owner_key(d, (uint32_t)i, 0, use_rank != 0);
-    for (uint32_t o = 0; o < 4; ++o) {
+    const uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (w > d.W0) return;
+    const uint64_t key = w < d.W0 ? d.owner[w] : 0ull;  // keys are >= 1 << 34
+    uint32_t instr = 0, rd = 0, rs1 = 0, rs2 = 0, jt = 0;
+    if (key != 0ull) {
+        const uint32_t o = (uint32_t)key & 3u, r = (uint32_t)(key >> 2);
+        const uint32_t nn = use_rank ? d.rinv[r] : r;
+        const GenNode N = gen_node(d, nn);
         Lane L;
         gen_lane(d, N, o, true, L);
-        if (!L.has || L.loc >= d.W0) continue;
-        if (d.owner[L.loc] != (k0 | o)) continue;
-        // virtual registers (> 63) and targets become global word indices; a
-        // virtual register past its program's words has no state (the
-        // single-program check is virt_ok's)
+        if (!L.has || L.loc != w) atomicOr(d.flags, EF_OOB);
         const uint32_t wn = N.wn;
         if (d.P > 1 && (((int32_t)L.rd > 63 && L.rd - 64u >= wn) || ((int32_t)L.rs1 > 63 && L.rs1 - 64u >= wn) ||
                         ((int32_t)L.rs2 > 63 && L.rs2 - 64u >= wn)))
             atomicOr(d.flags, EF_OOB);
-        d.w0_instr[L.loc] = L.word;
-        d.w0_rd[L.loc] = (int32_t)L.rd > 63 ? L.rd + L.wb : L.rd;
-        d.w0_rs1[L.loc] = (int32_t)L.rs1 > 63 ? L.rs1 + L.wb : L.rs1;
-        d.w0_rs2[L.loc] = (int32_t)L.rs2 > 63 ? L.rs2 + L.wb : L.rs2;
-        d.w0_jt[L.loc] = L.jt + L.wb;
+        instr = L.word;
+        rd = (int32_t)L.rd > 63 ? L.rd + L.wb : L.rd;
+        rs1 = (int32_t)L.rs1 > 63 ? L.rs1 + L.wb : L.rs1;
+        rs2 = (int32_t)L.rs2 > 63 ? L.rs2 + L.wb : L.rs2;
+        jt = L.jt + L.wb;
     }
+    d.w0_instr[w] = instr; d.w0_rd[w] = rd; d.w0_rs1[w] = rs1; d.w0_rs2[w] = rs2; d.w0_jt[w] = jt;
 }
 
 // ------------------------------------------ libstdc++ introsort, reproduced
@@ -1310,21 +1316,25 @@ __global__ void k_sort_insertion(const uint8_t *bound, uint64_t n, uint32_t *si,
     }
 }
 
-__global__ void k_sort_rank(const uint32_t *si, uint64_t n, uint32_t *rank) {
+__global__ void k_sort_rank(const uint32_t *si, uint64_t n, uint32_t *rank, uint32_t *rinv) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    rank[si[i]] = (uint32_t)i;
+    const uint32_t v = si[i];
+    rank[v] = (uint32_t)i;
+    rinv[i] = v;
 }
 
 // Elements of all-equal ranges go to their closed-form places (after the
 // plain ranks, which they overwrite).
 __global__ void k_sort_rank_uniform(const Seg *uni, uint32_t nuni, uint32_t total, const uint32_t *si,
-                                    uint32_t *rank) {
+                                    uint32_t *rank, uint32_t *rinv) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
     const Seg g = uni[seg_find(uni, nuni, e)];
     const uint32_t p = g.first + (e - g.off);
-    rank[si[p]] = uniform_final(g.first, g.last, p);
+    const uint32_t v = si[p], q = uniform_final(g.first, g.last, p);
+    rank[v] = q;
+    rinv[q] = v;  // (positions of an all-equal range are a permutation of it)
 }
 
 // ================================================================= optimize
