@@ -2871,6 +2871,13 @@ __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
     if (i0 < d.W0 && d.F) func_range(d, (uint32_t)i0, fs, fe);
     const uint32_t fsp = fe > fs && sym_swapped(d, fs) ? 1u : 0u;
     auto spill = [&](uint32_t v) -> uint32_t { return v >= fs && v < fe ? fsp : spill_of(d, v); };
+    // slots no word's range covers: the pad at W1 and, when the last word is
+    // unused, the reference's last slot W1 - 1 (W1 = last offset + 1)
+    auto zero = [&](uint32_t o) {
+        d.w1_instr[o] = 0; d.w1_rd[o] = 0; d.w1_rs1[o] = 0; d.w1_rs2[o] = 0; d.w1_jt[o] = 0;
+    };
+    if (i == 0) zero(d.W1);
+    if (i + 1 == d.W0 && !d.usedi[i] && d.ra_off[i] < d.W1) zero(d.ra_off[i]);
     if (i >= d.W0 || !d.usedi[i]) return;
     const int32_t base = (int32_t)d.ra_off[i];
     const uint32_t ins = d.w0_instr[i];
@@ -2901,19 +2908,35 @@ __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
         std_ = spill(rdv - 64u);
         ad = e & 0xFFu;
     }
-    if ((rdo >= 0 && (uint32_t)rdo >= d.W1) || (uint32_t)mo >= d.W1) { atomicOr(d.flags, EF_OOB); return; }
-    if (l1 >= 0) { d.w1_instr[l1] = (((st1 - 1u) * 4u) << 20) | 0x2003u; d.w1_rd[l1] = 5; }
-    if (l2 >= 0) { d.w1_instr[l2] = (((st2 - 1u) * 4u) << 20) | 0x2003u; d.w1_rd[l2] = 6; }
-    if (mo >= 0) {
-        d.w1_instr[mo] = ins; d.w1_rd[mo] = ad; d.w1_rs1[mo] = a1; d.w1_rs2[mo] = a2;
-        d.w1_jt[mo] = d.ra_off[d.w0_jt[i] < d.W0 ? d.w0_jt[i] : 0];
+    // The word's slots are [base, base + n) (its count in RaCountOp); the
+    // reference's stores land in them as below, later ones over earlier ones,
+    // and a slot none of them reaches stays zero.  Each slot of the range is
+    // written once, all five fields, so the arrays need no clearing.
+    const uint32_t n = 1u + (l1 >= 0 ? 1u : 0u) + (l2 >= 0 ? 1u : 0u) + (rdo >= 0 ? 1u : 0u);
+    const uint32_t end = (uint32_t)base + n;
+    if ((rdo >= 0 && (uint32_t)rdo >= end) || (uint32_t)mo >= end || end > d.W1) { atomicOr(d.flags, EF_OOB); return; }
+    uint32_t wi[4] = {0, 0, 0, 0}, wd[4] = {0, 0, 0, 0}, w1[4] = {0, 0, 0, 0}, w2[4] = {0, 0, 0, 0}, wj[4] = {0, 0, 0, 0};
+    auto at = [&](int32_t slot) -> uint32_t { return (uint32_t)(slot - base); };  // < n for every slot stored
+    if (l1 >= 0) { wi[at(l1)] = (((st1 - 1u) * 4u) << 20) | 0x2003u; wd[at(l1)] = 5; }
+    if (l2 >= 0) { wi[at(l2)] = (((st2 - 1u) * 4u) << 20) | 0x2003u; wd[at(l2)] = 6; }
+    {
+        const uint32_t k = at(mo);
+        wi[k] = ins; wd[k] = ad; w1[k] = a1; w2[k] = a2;
+        wj[k] = d.ra_off[d.w0_jt[i] < d.W0 ? d.w0_jt[i] : 0];
         if (d.w0_jt[i] >= d.W0) atomicOr(d.flags, EF_OOB);
     }
     if (rdo >= 0) {
+        const uint32_t k = at(rdo);
         uint32_t imm = (std_ - 1u) * (uint32_t)-4;
-        d.w1_instr[rdo] = 0x2023u | ((imm & 0x1Fu) << 7) | ((imm & 0xFEu) << 19);
-        d.w1_rs1[rdo] = ad;
+        wi[k] = 0x2023u | ((imm & 0x1Fu) << 7) | ((imm & 0xFEu) << 19);
+        wd[k] = 0; w1[k] = ad; w2[k] = 0; wj[k] = 0;
     }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+        if (k < n) {
+            const uint32_t o = (uint32_t)base + k;
+            d.w1_instr[o] = wi[k]; d.w1_rd[o] = wd[k]; d.w1_rs1[o] = w1[k]; d.w1_rs2[o] = w2[k]; d.w1_jt[o] = wj[k];
+        }
 }
 
 // fix_func_tab (:1816-1837) against an offset array; pend[p] is the end of
