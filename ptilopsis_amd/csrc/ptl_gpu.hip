@@ -1375,42 +1375,77 @@ constexpr uint32_t OPT_RANGE = 256 * 8;
 // after every wave has read it.  With a single flag, a wave still to read it
 // could see the next round's clear, leave the loop and drop its words from
 // the later rounds, and the launch could then report a fixpoint early.
+// Thread t sweeps its own 8 consecutive words, highest first, so a chain of
+// words naming the word before (the usual shape) settles inside one sweep of
+// its thread rather than one link per round; the range is kept in LDS
+// transposed (word k at (k % 8) * 256 + k / 8), so that the threads' k-th words
+// sit in consecutive banks.
+DEV uint32_t opt_tp(uint32_t k) { return (k & 7u) * 256u + (k >> 3); }
+
 __global__ __launch_bounds__(256) void k_opt_iter(Dev d, uint32_t *changed) {
+    static_assert(OPT_RANGE == 256 * 8, "8 words per thread");
     __shared__ int local[2];
     __shared__ uint32_t s_rd[OPT_RANGE], s_r1[OPT_RANGE], s_r2[OPT_RANGE];
     __shared__ uint8_t s_used[OPT_RANGE], s_st[OPT_RANGE];
     const uint64_t b0 = (uint64_t)blockIdx.x * OPT_RANGE;
     const uint64_t b1 = b0 + OPT_RANGE < d.W0 ? b0 + OPT_RANGE : d.W0;
     const uint32_t nr = (uint32_t)(b1 - b0);
-    for (uint32_t k = threadIdx.x; k < nr; k += 256) {
-        s_rd[k] = d.w0_rd[b0 + k]; s_r1[k] = d.w0_rs1[b0 + k]; s_r2[k] = d.w0_rs2[b0 + k];
-        s_used[k] = d.used[b0 + k]; s_st[k] = d.sstore[b0 + k];
+    const uint32_t t = threadIdx.x, k0 = 8u * t;
+    if (k0 + 8u <= nr) {
+        // this thread's 8 words with two 16-byte loads per field (b0 and k0 are multiples of 8)
+        const uint4 *rd4 = reinterpret_cast<const uint4 *>(d.w0_rd + b0 + k0);
+        const uint4 *r14 = reinterpret_cast<const uint4 *>(d.w0_rs1 + b0 + k0);
+        const uint4 *r24 = reinterpret_cast<const uint4 *>(d.w0_rs2 + b0 + k0);
+        const uint4 x0 = rd4[0], x1 = rd4[1], y0 = r14[0], y1 = r14[1], z0 = r24[0], z1 = r24[1];
+        const uint2 u8 = *reinterpret_cast<const uint2 *>(d.used + b0 + k0);
+        const uint2 s8 = *reinterpret_cast<const uint2 *>(d.sstore + b0 + k0);
+        const uint32_t xr[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+        const uint32_t yr[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+        const uint32_t zr[8] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w};
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            const uint32_t e = j * 256u + t;
+            s_rd[e] = xr[j]; s_r1[e] = yr[j]; s_r2[e] = zr[j];
+            s_used[e] = (uint8_t)((j < 4 ? u8.x : u8.y) >> (8u * (j & 3u)));
+            s_st[e] = (uint8_t)((j < 4 ? s8.x : s8.y) >> (8u * (j & 3u)));
+        }
+    } else {
+        for (uint32_t j = 0; j < 8; ++j) {
+            const uint32_t k = k0 + j;
+            if (k >= nr) break;
+            const uint32_t e = j * 256u + t;
+            s_rd[e] = d.w0_rd[b0 + k]; s_r1[e] = d.w0_rs1[b0 + k]; s_r2[e] = d.w0_rs2[b0 + k];
+            s_used[e] = d.used[b0 + k]; s_st[e] = d.sstore[b0 + k];
+        }
     }
-    auto used_of = [&](uint32_t x) -> bool { return (uint64_t)x - b0 < nr ? s_used[x - b0] != 0 : d.used[x] != 0; };
-    auto store_of = [&](uint32_t x) -> bool { return (uint64_t)x - b0 < nr ? s_st[x - b0] != 0 : d.sstore[x] != 0; };
+    auto used_of = [&](uint32_t x) -> bool { return (uint64_t)x - b0 < nr ? s_used[opt_tp(x - (uint32_t)b0)] != 0 : d.used[x] != 0; };
+    auto store_of = [&](uint32_t x) -> bool { return (uint64_t)x - b0 < nr ? s_st[opt_tp(x - (uint32_t)b0)] != 0 : d.sstore[x] != 0; };
     bool cross = false;  // a clear outside the range, or a destination outside it
     bool settled = false;
     for (int round = 0; round < 64; ++round) {
         if (threadIdx.x == 0) local[round & 1] = 0;
         __syncthreads();
         bool loc = false;
-        for (int32_t k = (int32_t)nr - 1 - (int32_t)threadIdx.x; k >= 0; k -= 256) {
-            const uint32_t rd = s_rd[k];
+        for (int32_t j = 7; j >= 0; --j) {
+            const uint32_t k = k0 + (uint32_t)j;
+            if (k >= nr) continue;
+            const uint32_t e = (uint32_t)j * 256u + t;
+            const uint32_t rd = s_rd[e];
             if (!((int32_t)rd > 63)) continue;
             const uint32_t v = rd - 64u;
             if (v >= d.W0) continue;
             cross |= (uint64_t)v - b0 >= nr;
             if (used_of(v)) continue;
-            const int32_t a = (int32_t)(s_r1[k] - 64u), b = (int32_t)(s_r2[k] - 64u);
+            const int32_t a = (int32_t)(s_r1[e] - 64u), b = (int32_t)(s_r2[e] - 64u);
             if (a >= 0 && (uint32_t)a < d.W0 && !store_of((uint32_t)a) && used_of((uint32_t)a)) {
                 const bool in = (uint64_t)(uint32_t)a - b0 < nr;
-                if (in) s_used[(uint32_t)a - b0] = 0;
+                if (in) s_used[opt_tp((uint32_t)a - (uint32_t)b0)] = 0;
                 d.used[a] = 0;
                 loc |= in; cross |= !in;
             }
             if (b >= 0 && (uint32_t)b < d.W0 && !store_of((uint32_t)b) && used_of((uint32_t)b)) {
                 const bool in = (uint64_t)(uint32_t)b - b0 < nr;
-                if (in) s_used[(uint32_t)b - b0] = 0;
+                if (in) s_used[opt_tp((uint32_t)b - (uint32_t)b0)] = 0;
                 d.used[b] = 0;
                 loc |= in; cross |= !in;
             }
