@@ -87,6 +87,18 @@ DEV uint32_t prog_of(const uint32_t *base, uint32_t P, uint32_t i) {
     return lo;
 }
 
+// The same where neighbouring lanes look up neighbouring elements (node- or
+// word-parallel kernels): one search for the wave's first active lane, on a
+// wave-uniform index, then each lane steps from there to its own program (the
+// programs are long against a wave, so mostly no step).
+DEV uint32_t prog_of_w(const uint32_t *base, uint32_t P, uint32_t i) {
+    const uint32_t i0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)i);
+    uint32_t p = prog_of(base, P, i0);
+    while (p > 0 && base[p] > i) --p;
+    while (p + 1 < P && base[p + 1] <= i) ++p;
+    return p;
+}
+
 DEV uint32_t ffz_gcc(uint64_t v) {
     // include/utils.hpp:25-45 as compiled by GCC: __builtin_ffs(int) of ~val sees 32 bits.
     uint32_t low = (uint32_t)(~v);
@@ -396,7 +408,7 @@ __global__ __launch_bounds__(256) void k_pre12(Dev d) {
     if (!in) return;
     // does the node's 8-node vector (counted from its program's first node)
     // hold a call argument list?
-    const uint32_t prg = prog_of(d.pn, d.P, (uint32_t)i);
+    const uint32_t prg = prog_of_w(d.pn, d.P, (uint32_t)i);
     const uint32_t v0 = d.pn[prg] + (((uint32_t)i - d.pn[prg]) & ~7u), v1 = min(v0 + 8u, d.pn[prg + 1]);
     bool block_has_list = false;
     for (uint32_t j = v0; j < v1; ++j) block_has_list |= c_pareas[d.pt[j]] == RV_CALL_ARGS;
@@ -421,7 +433,7 @@ __global__ __launch_bounds__(256) void k_pre12(Dev d) {
 __global__ __launch_bounds__(256) void k_pre3(Dev d) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= d.n || d.prt[i] != DT_FLOAT) return;
-    const uint32_t b = d.pn[prog_of(d.pn, d.P, (uint32_t)i)];  // vectors count from the program's first node
+    const uint32_t b = d.pn[prog_of_w(d.pn, d.P, (uint32_t)i)];  // vectors count from the program's first node
     uint64_t cur = i;
     for (;;) {
         int32_t p = d.par[cur];
@@ -494,7 +506,7 @@ __global__ void k_globalize_parents(int32_t *par, uint64_t n, const uint32_t *pn
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t p = par[i];
-    if (p >= 0) par[i] = p + (int32_t)pn[prog_of(pn, P, (uint32_t)i)];
+    if (p >= 0) par[i] = p + (int32_t)pn[prog_of_w(pn, P, (uint32_t)i)];
 }
 
 // Word base of every program: the exclusive count scan at its first node (the
@@ -548,7 +560,7 @@ struct GenNode {
 DEV GenNode gen_node(const Dev &d, uint32_t nn) {
     GenNode N;
     N.nn = nn;
-    N.prg = prog_of(d.pn, d.P, nn);
+    N.prg = prog_of_w(d.pn, d.P, nn);
     N.wb = d.pw[N.prg]; N.wn = d.pw[N.prg + 1] - N.wb;
     N.fb = d.pf[N.prg]; N.Fp = d.pf[N.prg + 1] - N.fb;
     N.ty = d.ty8[nn]; N.rt = d.rt8[nn];
@@ -2887,14 +2899,6 @@ DEV uint32_t spill_of(const Dev &d, uint32_t v) {
     return sym_swapped(d, d.fstarts[lo]) ? 1u : 0u;
 }
 
-// The function holding word w (last start <= w) and its word range [fs, fe).
-DEV void func_range(const Dev &d, uint32_t w, uint32_t &fs, uint32_t &fe) {
-    uint32_t lo = 0, hi = d.F;
-    while (hi - lo > 1) { uint32_t m = (lo + hi) / 2; if (d.fstarts[m] <= w) lo = m; else hi = m; }
-    fs = d.fstarts[lo];
-    fe = lo + 1 < d.F ? d.fstarts[lo + 1] : 0xFFFFFFFFu;
-}
-
 // Emit (:1552-1697).
 __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -2902,10 +2906,28 @@ __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
     // word's operands are mostly registers of its own function, and the
     // wave's words mostly of one function (spill_of of any other register)
     const uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~63u);
-    uint32_t fs = 0, fe = 0;
-    if (i0 < d.W0 && d.F) func_range(d, (uint32_t)i0, fs, fe);
+    uint32_t fi = 0, fs = 0, fe = 0;
+    if (i0 < d.W0 && d.F) {
+        uint32_t lo = 0, hi = d.F;  // last start <= i0
+        while (hi - lo > 1) { const uint32_t m = (lo + hi) / 2; if (d.fstarts[m] <= (uint32_t)i0) lo = m; else hi = m; }
+        fi = lo;
+        fs = d.fstarts[lo];
+        fe = lo + 1 < d.F ? d.fstarts[lo + 1] : 0xFFFFFFFFu;
+    }
     const uint32_t fsp = fe > fs && sym_swapped(d, fs) ? 1u : 0u;
-    auto spill = [&](uint32_t v) -> uint32_t { return v >= fs && v < fe ? fsp : spill_of(d, v); };
+    // the function of this lane's own word, stepping on from the wave's (a
+    // wave's words span few functions), for a lane past the wave's function
+    uint32_t ls = fs, le = fe, lsp = fsp;
+    if (i < d.W0 && d.F && (uint32_t)i >= fe) {
+        uint32_t li = fi + 1;
+        while (li + 1 < d.F && d.fstarts[li + 1] <= (uint32_t)i) ++li;
+        ls = d.fstarts[li];
+        le = li + 1 < d.F ? d.fstarts[li + 1] : 0xFFFFFFFFu;
+        lsp = sym_swapped(d, ls) ? 1u : 0u;
+    }
+    auto spill = [&](uint32_t v) -> uint32_t {
+        return v >= ls && v < le ? lsp : v >= fs && v < fe ? fsp : spill_of(d, v);
+    };
     // slots no word's range covers: the pad at W1 and, when the last word is
     // unused, the reference's last slot W1 - 1 (W1 = last offset + 1)
     auto zero = [&](uint32_t o) {
