@@ -1342,7 +1342,12 @@ __global__ void k_sort_rank_uniform(const Seg *uni, uint32_t nuni, uint32_t tota
                                     uint32_t *rank, uint32_t *rinv) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= total) return;
-    const Seg g = uni[seg_find(uni, nuni, e)];
+    // the range of the wave's first active element by one search on a
+    // wave-uniform index, then each lane steps on to its own (a range has more
+    // than 16 elements, so a wave spans at most five)
+    uint32_t si0 = seg_find(uni, nuni, (uint32_t)__builtin_amdgcn_readfirstlane((int)e));
+    while (si0 + 1 < nuni && uni[si0 + 1].off <= e) ++si0;
+    const Seg g = uni[si0];
     const uint32_t p = g.first + (e - g.off);
     const uint32_t v = si[p], q = uniform_final(g.first, g.last, p);
     rank[v] = q;
