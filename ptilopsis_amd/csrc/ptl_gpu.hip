@@ -2892,7 +2892,8 @@ struct RaCountOp {
     DEV uint32_t sw(uint32_t r) const { return ((int32_t)r > 63 && r - 64u < d.W0 && sym_swapped(d, r - 64u)) ? 1u : 0u; }
     DEV uint32_t value(uint64_t i) const {
         if (!d.usedi[i]) return 0;
-        return 1u + sw(d.w0_rs1[i]) + sw(d.w0_rs2[i]) + sw(d.w0_rd[i]);
+        const uint4 w = d.w0_f4[i];  // rd, rs1, rs2, instruction
+        return 1u + sw(w.y) + sw(w.z) + sw(w.x);
     }
     DEV void emit(uint64_t i, uint32_t excl, uint32_t) const { d.ra_off[i] = excl; }
 };
@@ -2942,8 +2943,9 @@ __global__ __launch_bounds__(256) void k_ra_emit(Dev d) {
     if (i + 1 == d.W0 && !d.usedi[i] && d.ra_off[i] < d.W1) zero(d.ra_off[i]);
     if (i >= d.W0 || !d.usedi[i]) return;
     const int32_t base = (int32_t)d.ra_off[i];
-    const uint32_t ins = d.w0_instr[i];
-    const uint32_t r1 = d.w0_rs1[i], r2 = d.w0_rs2[i], rdv = d.w0_rd[i];
+    const uint4 wf = d.w0_f4[i];  // rd, rs1, rs2, instruction
+    const uint32_t ins = wf.w;
+    const uint32_t r1 = wf.y, r2 = wf.z, rdv = wf.x;
     int32_t l1 = -1, l2 = -1, rdo = -1;
     uint32_t a1 = r1, a2 = r2, ad = rdv, st1 = 0, st2 = 0, std_ = 0;
     if (((int32_t)r1 > 63 && r1 - 64u >= d.W0) || ((int32_t)r2 > 63 && r2 - 64u >= d.W0) ||
