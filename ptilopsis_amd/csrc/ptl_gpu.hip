@@ -1179,9 +1179,11 @@ __global__ __launch_bounds__(256) void k_sort_local(const Seg *loc, const int32_
                 cl += b & 1u; cr += b >> 1;
             }
         }
-        uint32_t totL, totR;
-        uint32_t rl = block_excl_scan(cl, tmp, totL);
-        uint32_t rr = block_excl_scan(cr, tmp, totR);
+        // both counts in one scan (each at most SORT_LOCAL, so 16 bits apiece)
+        uint32_t tot2;
+        const uint32_t x2 = block_excl_scan(cl | (cr << 16), tmp, tot2);
+        const uint32_t totL = tot2 & 0xFFFFu, totR = tot2 >> 16;
+        uint32_t rl = x2 & 0xFFFFu, rr = x2 >> 16;
         const uint32_t rl0 = rl;
         {
             uint32_t s = s0;
