@@ -1875,14 +1875,16 @@ __global__ void k_walk_merge(uint32_t *gmin, uint32_t *gmax, uint64_t stride, ui
 // write and resets.  Every lookup and mark of the step comes after every
 // operand read of the step, as in the vector loop.  A mark on a register outside
 // lane 0's own function cannot be kept in its cache: the program then takes the
-// exact serial path.
+// exact serial path.  The first `solo` programs of the list (the longest) have
+// a wave each, in slot 0, so that no other program's rarer paths (a cache
+// miss, a mark) hold up their steps.
 template <uint32_t WALK_RING>
-__global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, uint32_t np) {
+__global__ __launch_bounds__(64) void k_ra_group(Dev d, const uint32_t *plist, uint32_t np, uint32_t solo) {
     __shared__ uint32_t cache[WALK_CACHE][64];
     __shared__ uint4 words[WALK_CHUNK][64];
     __shared__ int32_t Sv[8][64];
     const uint32_t lane = threadIdx.x, slot = lane >> 3, k = lane & 7u;
-    const uint32_t pi = blockIdx.x * 8u + slot;
+    const uint32_t pi = blockIdx.x < solo ? (slot == 0 ? blockIdx.x : np) : solo + (blockIdx.x - solo) * 8u + slot;
     const uint32_t prg = pi < np ? plist[pi] : 0xFFFFFFFFu;
     const uint32_t fb = prg != 0xFFFFFFFFu ? d.pf[prg] : 0u;
     const uint32_t nf = prg != 0xFFFFFFFFu ? d.pf[prg + 1] - fb : 0u;
