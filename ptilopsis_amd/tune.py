@@ -91,7 +91,7 @@ def main() -> int:
         2: ("program_4m", [W.sized_program(0x5EED0011, 4_000_000, 5000)]),
         3: ("large_16m", [W.large_16m()]),
     }
-    ring_work_3 = ("stress_1024", W.stress_1024())  # the walks bound this batch; they hide behind the lockstep in C2
+    ring_work_3 = ("stress_1024", W.stress_1024())  # the walks bound this batch; in the one 17M-node program they hide behind the lockstep
     print(f"workloads generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     ctx = Context(0)
     values = {k: [v[1]] * len(BUCKETS) for k, v in KNOBS.items()}
